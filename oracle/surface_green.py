@@ -1,0 +1,129 @@
+"""Lead surface Green's function and self-energy (oracle only).
+
+Restates ``dpnegf/negf/surface_green.py``:
+
+* ``lopez_sancho``   -> ``_surface_green_scipy_core`` ``:169-226`` (the numba
+  core ``:102-159`` performs the same arithmetic).  Returns the iteration
+  count and residual as well, for flop accounting.
+* ``surface_green``  -> ``:229-279`` (dispatch; GEP branch ``:242-244``).
+* ``selfEnergy``     -> ``:9-91``.
+* ``calcg0``         -> ``:284-347`` plus ``iterative_gf_numpy`` ``:349-365``
+  (generalised-eigenproblem route, used as an independent cross-check).
+"""
+import logging
+
+import numpy as np
+import scipy.linalg as sla
+import torch
+
+log = logging.getLogger(__name__)
+
+LS_STOP = 1.0e-40      # surface_green.py:208
+LS_RESID_FAIL = 3.0e-5  # surface_green.py:214
+LS_RESID_WARN = 1.0e-8  # surface_green.py:216
+LS_MAX_ITER = 101       # surface_green.py:222
+
+
+def lopez_sancho(H, h01, S, s01, z, stop=LS_STOP):
+    """Lopez-Sancho decimation; returns ``(g_s, iterations, residual)``.
+
+    ``z`` already carries ``+ i*eta_lead``.  Raises ``ArithmeticError`` on a
+    failed Dyson residual and ``RuntimeError`` past 100 iterations, exactly as
+    ``surface_green.py:206-224``.
+    """
+    n = H.shape[0]
+    h10 = h01.conj().T
+    s10 = s01.conj().T
+    alpha = h10 - z * s10
+    beta = h01 - z * s01
+    eps = H.astype(np.complex128).copy()
+    eps_s = eps.copy()
+    zS = z * S
+    it = 0
+    while True:
+        it += 1
+        rhs = np.concatenate([alpha, beta], axis=1)
+        x = sla.solve(zS - eps, rhs, check_finite=False)
+        xa, xb = x[:, :n], x[:, n:]
+        b_xa = beta @ xa
+        eps = eps + alpha @ xb + b_xa
+        eps_s = eps_s + b_xa
+        alpha = alpha @ xa
+        beta = beta @ xb
+        if np.max(np.abs(alpha) + np.abs(beta)) < stop:
+            gs = np.linalg.inv(zS - eps_s)
+            t = zS - H - (z * s01 - h01) @ gs @ (z * s10 - h10)
+            resid = float(np.max(np.abs(t @ gs - np.eye(n))))
+            if resid >= LS_RESID_FAIL:
+                raise ArithmeticError("Criteria not met. Please check output...")
+            if resid > LS_RESID_WARN:
+                log.warning("Lopez-scheme not-so-well converged at E = %.4f eV: %g",
+                            np.real(z), resid)
+            return gs, it, resid
+        if it >= LS_MAX_ITER:
+            raise RuntimeError("Lopez-scheme not converged.")
+
+
+def iterative_gf_numpy(ee, gs, h00, h01, s00, s01, iter=1):
+    """``surface_green.py:349-365``."""
+    for _ in range(iter):
+        m = ee * s00 - h00 - (ee * s01 - h01) @ gs @ (ee * s01.conj().T - h01.conj().T)
+        gs = np.linalg.pinv(m)
+    return gs
+
+
+def calcg0(ee, h00, s00, h01, s01):
+    """GEP (transfer-matrix) surface Green's function, ``surface_green.py:284-347``."""
+    n = h00.shape[0]
+    ee = ee.real + max(ee.imag, 1e-8) * 1.0j
+    a = np.zeros((2 * n, 2 * n), dtype=np.complex128)
+    b = np.zeros((2 * n, 2 * n), dtype=np.complex128)
+    a[:n, :n] = ee * s00 - h00
+    a[:n, n:] = -np.eye(n)
+    a[n:, :n] = h01.conj().T - ee * s01.conj().T
+    b[:n, :n] = h01 - ee * s01
+    b[n:, n:] = np.eye(n)
+    ev, vec = sla.eig(a=a, b=b)
+    keep = np.where(np.abs(ev) < 1.0)[0]
+    ev, vec = ev[keep], vec[:n, keep].T
+    vec = np.diag(1.0 / np.sqrt(np.diag(vec @ vec.conj().T))) @ vec
+    ep = vec.T
+    fp = ep @ np.diag(ev) @ np.linalg.inv(ep.conj().T @ ep) @ ep.conj().T
+    g00 = np.linalg.inv(ee * s00 - h00 - (h01 - ee * s01) @ fp)
+    return iterative_gf_numpy(ee, g00, h00, h01, s00, s01, iter=3)
+
+
+def surface_green(H, h01, S, s01, ee, method='Lopez-Sancho', numba_jit=None):
+    """Dispatcher of ``surface_green.py:229-279`` (numba absent: scipy core)."""
+    if method == 'GEP':
+        return calcg0(ee, H, S, h01, s01)
+    return lopez_sancho(H, h01, S, s01, ee)[0]
+
+
+def _np(x):
+    if isinstance(x, torch.Tensor):
+        return x.detach().cpu().numpy()
+    return np.asarray(x)
+
+
+def selfEnergy(hL, hLL, sL, sLL, ee, hDL=None, sDL=None, etaLead=1e-8, Bulk=False,
+               E_ref=0.0, dtype=np.complex128, device='cpu', method='Lopez-Sancho',
+               numba_jit=None):
+    """``surface_green.py:9-91``: returns ``(Sigma, g_s)`` as complex128 tensors.
+
+    Couplings use the real-axis (no ``etaLead``) shifted energy; only the
+    decimation sees ``+ i*etaLead`` (``:83-86``).
+    """
+    hL, sL, hLL, sLL = _np(hL), _np(sL), _np(hLL), _np(sLL)
+    e = (np.asarray(_np(ee), dtype=dtype) + _np(E_ref)).item()
+    if hDL is None:
+        esh = e * sL - hL
+        gs = surface_green(hL, hLL, sL, sLL, e + 1j * etaLead, method, numba_jit)
+        sig = np.linalg.inv(gs) if Bulk else esh - np.linalg.inv(gs)
+    else:
+        hDL, sDL = _np(hDL), _np(sDL)
+        b = hDL.shape[1]
+        gs = surface_green(hL, hLL, sL, sLL, e + 1j * etaLead, method, numba_jit)
+        sig = (e * sDL - hDL) @ gs[:b, :b] @ (e * sDL.conj().T - hDL.conj().T)
+    return (torch.tensor(sig, dtype=torch.complex128, device=device),
+            torch.tensor(gs, dtype=torch.complex128, device=device))
