@@ -1,0 +1,166 @@
+"""Benchmark: (E,k) points/s of the NEGF transmission path on one MI355X.
+
+Workload (the headline configuration): square-lattice ribbon, 1 orbital/site,
+t = -2.7 eV, width 48 (block 48), 128 device layers with Anderson disorder
+U[-0.25, 0.25] eV (seed 0), clean semi-infinite leads, 4096 energies on
+[-3, 3] eV, eta_lead = eta_device = 1e-6, zero bias, complex128.
+
+One step = the whole job for one energy grid with the Hamiltonian already in
+HBM: Lopez-Sancho self-energies of both leads at all 4096 energies, then the
+fused RGF transmission sweep.  ``value`` = 4096 / step time.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RGF (E,k) points/s + adjoint: mixed vs complex128 on 1 GPU at |ΔT|≤1e-6"
+PEAK_FP64 = 78.6e12
+PEAK_FP32 = 157.3e12
+
+
+def rgf_flops_per_point(sizes, nL, nR):
+    """Conventional real flops of the fused transmission sweep per point
+    (8 per complex multiply-add, 8 n^3 per complex inverse)."""
+    n0 = sizes[0]
+    f = 8.0 * n0 ** 3  # g_0
+    for q in range(len(sizes) - 1):
+        nq, n1 = sizes[q], sizes[q + 1]
+        f += 8.0 * (n0 + nq) * nq * n1  # [P; g] U
+        f += 8.0 * n1 * nq * n1         # L Y
+        f += 8.0 * n1 ** 3              # inverse
+        f += 8.0 * n0 * n1 * n1         # P g
+    f += 8.0 * nL * nR * nR + 8.0 * nL * nL * nR  # trace
+    return f
+
+
+def build_workload(dev, W=48, K=128, nE=4096, seed=0):
+    from dpnegf_amd import workloads as wl
+    s = wl.ribbon(W=W, K=K, seed=seed)
+    E = torch.linspace(-3.0, 3.0, nE, dtype=torch.float64)
+    return s, E
+
+
+def cpu_baseline(s, E, eta, n_sample):
+    """Oracle (complex128 torch/scipy restatement of the reference) on a
+    bounded, evenly spaced sample of the energy grid."""
+    from oracle import device as odev, lead as olead, rgf as orgf, surface_green as osg
+    ncores = os.cpu_count() or 1
+    torch.set_num_threads(ncores)
+    idx = np.linspace(0, len(E) - 1, n_sample).round().astype(int)
+    t0 = time.perf_counter()
+    for i in idx:
+        e = float(E[i])
+        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e), hDL=l[2], sDL=l[5],
+                              etaLead=eta)[0] for l in (s.lead_L, s.lead_R)]
+        ans = orgf.recursive_gf(torch.tensor(e, dtype=torch.complex128), s.hl, s.hd, s.hu, s.sd,
+                                s.su, s.sl, sig[0], sig[1], eta=eta, keep_gr_left=False)
+        odev.transmission(ans[0], olead.sigma2gamma(sig[0]), olead.sigma2gamma(sig[1]))
+    dt = time.perf_counter() - t0
+    return {"value": n_sample / dt, "unit": "(E,k) points/s", "cores": ncores, "kind": "port",
+            "sample": f"{n_sample} of {len(E)} energies, evenly spaced; Sigma_L, Sigma_R "
+                      f"(scipy Lopez-Sancho) + RGF (torch) + T per energy, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--energies", type=int, default=4096)
+    ap.add_argument("--cpu-sample", type=int, default=24)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box in this setup has a single MI355X and "
+              "the (E,k) points shard trivially across GPUs (see DESIGN.md); run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from dpnegf_amd.negf.recursive_green_cal import BTDSystems, rgf_transmission
+    from dpnegf_amd.negf.surface_green import self_energy_batched
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    eta = 1e-6
+    s_cpu, E_cpu = build_workload(dev, nE=args.energies)
+    s = s_cpu.to(dev)
+    E = E_cpu.to(dev).to(torch.complex128)
+    nE = E.shape[0]
+    bs = BTDSystems([(s.hd, s.hu, s.hl, s.sd, s.su, s.sl)], dev)
+    leads = [tuple(x.to(dev) for x in s.lead_L)], [tuple(x.to(dev) for x in s.lead_R)]
+    z = (E + 1j * eta).contiguous()
+    n0 = bs.sizes[0]
+    T = torch.empty(nE, dtype=torch.float64, device=dev)
+    status = torch.zeros(nE, dtype=torch.int32, device=dev)
+
+    def step(ev=None):
+        sL = self_energy_batched(leads[0], E, eta, raise_on_error=False).sigma
+        sR = self_energy_batched(leads[1], E, eta, raise_on_error=False).sigma
+        if ev is not None:
+            ev[0].record()
+        rgf_transmission(bs, z, sL, sR, precision=args.precision, status=status, out=T)
+        if ev is not None:
+            ev[1].record()
+        return sL, sR
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(events[k])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rgf_ms = float(np.mean([a.elapsed_time(b) for a, b in events]))
+    ms_per_step = 1e3 * dt / args.steps
+    points_per_s = nE / (dt / args.steps)
+    flops = rgf_flops_per_point(bs.sizes, n0, bs.sizes[-1])
+    peak = PEAK_FP64 if args.precision == "fp64" else PEAK_FP32
+    achieved = flops * nE / (rgf_ms * 1e-3)
+    out = {
+        "metric": METRIC,
+        "value": round(points_per_s, 3),
+        "unit": "(E,k) points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"ribbon W=48 K=128 Anderson U[-0.25,0.25] seed 0, {nE} energies "
+                               f"in [-3,3] eV, eta=1e-6; step = Sigma_L + Sigma_R (Lopez-Sancho) "
+                               f"+ fused RGF transmission",
+                   "points_per_step": nE, "precision": args.precision},
+        "rgf_kernel_ms": round(rgf_ms, 4),
+        "rgf_points_per_s": round(nE / (rgf_ms * 1e-3), 3),
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "kernel": "rgf_transmission_kernel",
+                     "flops_per_point": flops},
+        "T_checksum": float(T.sum().item()),
+        "status_nonzero": int((status != 0).sum().item()),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(s_cpu, E_cpu, eta, args.cpu_sample)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,21 @@
+#!/bin/bash
+# Builds libdpnegf_amd.so for gfx950 in-tree (dpnegf_amd/_lib/).
+set -euo pipefail
+here="$(cd "$(dirname "$0")" && pwd)"
+out="$here/../_lib"
+mkdir -p "$out" "$here/../_build"
+FLAGS="--offload-arch=gfx950 -O3 -std=c++17 -fPIC -Xclang -target-feature -Xclang -packed-fp32-ops -I$here -I$here/../../include"
+objs=()
+for f in rgf_transmission.hip lead_self_energy.hip rgf_full.hip capi.cpp; do
+  o="$here/../_build/${f%.*}.o"
+  if [[ "$f" == *.cpp ]]; then
+    hipcc $FLAGS -x hip -c "$here/$f" -o "$o" &
+  else
+    hipcc $FLAGS -c "$here/$f" -o "$o" &
+  fi
+  objs+=("$o")
+done
+wait
+hipcc --offload-arch=gfx950 -shared -fPIC "${objs[@]}" -o "$out/libdpnegf_amd.so.tmp"
+mv "$out/libdpnegf_amd.so.tmp" "$out/libdpnegf_amd.so"
+echo "built $out/libdpnegf_amd.so"

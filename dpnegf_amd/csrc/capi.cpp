@@ -1,0 +1,179 @@
+// extern "C" boundary of libdpnegf_amd.  Plain pointers and sizes only; every
+// data pointer is device memory unless its name starts with h_.  Entry points
+// return 0 on success and a negative code on failure; dpnegf_last_error()
+// gives the message.  Nothing here allocates device memory: workspaces are
+// passed in by the caller.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "dpnegf_amd.h"
+#include "rgf_full.hpp"
+#include "rgf_kernels.hpp"
+
+using negf::cx;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+  return fail(DPNEGF_ERR_HIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+int64_t arena_elems(int K, const int32_t* n) {
+  int64_t s = 0;
+  for (int q = 0; q < K; ++q) {
+    s += 2 * (int64_t)n[q] * n[q];
+    if (q + 1 < K) s += 4 * (int64_t)n[q] * n[q + 1];
+  }
+  return s;
+}
+
+int check_blocks(int K, const int32_t* h_bsz, int nmax_allowed, int* nmax_out) {
+  if (K < 1) return fail(DPNEGF_ERR_ARG, "K must be >= 1 (got %d)", K);
+  if (!h_bsz) return fail(DPNEGF_ERR_ARG, "h_bsz is null");
+  int m = 0;
+  for (int q = 0; q < K; ++q) {
+    if (h_bsz[q] < 1) return fail(DPNEGF_ERR_ARG, "block %d has size %d", q, h_bsz[q]);
+    if (h_bsz[q] > m) m = h_bsz[q];
+  }
+  if (m > nmax_allowed)
+    return fail(DPNEGF_ERR_UNSUPPORTED, "block size %d exceeds this entry point's limit %d", m,
+                nmax_allowed);
+  *nmax_out = m;
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int dpnegf_abi_version(void) { return DPNEGF_ABI_VERSION; }
+
+const char* dpnegf_last_error(void) { return g_err.c_str(); }
+
+int64_t dpnegf_arena_elems(int K, const int32_t* h_bsz) {
+  if (K < 1 || !h_bsz) return -1;
+  return arena_elems(K, h_bsz);
+}
+
+int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h_bsz,
+                            const int32_t* bsz, const void* arena, int64_t sys_stride, int nsys,
+                            const int32_t* sys_idx, int P, const void* z, const void* sigL, int nL,
+                            const void* sigR, int nR, int ortho, double* T, int32_t* status) {
+  int nmax = 0;
+  if (int r = check_blocks(K, h_bsz, DPNEGF_TRANSMISSION_NMAX, &nmax)) return r;
+  if (P < 0) return fail(DPNEGF_ERR_ARG, "P must be >= 0");
+  if (P == 0) return 0;
+  if (!bsz || !arena || !z || !sigL || !sigR || !T)
+    return fail(DPNEGF_ERR_ARG, "null device pointer");
+  if (nL < 1 || nL > h_bsz[0])
+    return fail(DPNEGF_ERR_ARG, "left self-energy size %d not in [1, %d]", nL, h_bsz[0]);
+  if (nR < 1 || nR > h_bsz[K - 1])
+    return fail(DPNEGF_ERR_ARG, "right self-energy size %d not in [1, %d]", nR, h_bsz[K - 1]);
+  if (nsys < 1) return fail(DPNEGF_ERR_ARG, "nsys must be >= 1");
+  if (nsys > 1 && !sys_idx) return fail(DPNEGF_ERR_ARG, "sys_idx required when nsys > 1");
+  if (nsys > 1 && sys_stride < arena_elems(K, h_bsz))
+    return fail(DPNEGF_ERR_ARG, "sys_stride smaller than one system's arena");
+  if (precision != DPNEGF_FP64 && precision != DPNEGF_MIXED)
+    return fail(DPNEGF_ERR_ARG, "unknown precision %d", precision);
+  negf::RgfArgs a;
+  a.K = K; a.bsz = bsz; a.arena = (const cx<double>*)arena; a.sys_stride = sys_stride;
+  a.sys_idx = sys_idx; a.P = P; a.z = (const cx<double>*)z;
+  a.sigL = (const cx<double>*)sigL; a.nL = nL; a.sigR = (const cx<double>*)sigR; a.nR = nR;
+  a.T = T; a.status = status;
+  hipError_t e;
+  if (precision == DPNEGF_FP64)
+    e = negf::launch_rgf_transmission<double>(a, nmax, ortho != 0, (hipStream_t)stream);
+  else
+    e = negf::launch_rgf_transmission_mixed(a, nmax, ortho != 0, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "rgf_transmission launch");
+  return 0;
+}
+
+int dpnegf_lead_self_energy(void* stream, int n, int nred, int mode, const void* lead,
+                            int64_t lead_stride, int nsys, const int32_t* sys_idx, int P,
+                            const void* e, double eta, double stop, void* eps_ws, void* gs,
+                            void* sigma, int ortho, int32_t* iters, int32_t* status,
+                            double* resid) {
+  if (n < 1 || nred < 1) return fail(DPNEGF_ERR_ARG, "bad lead sizes n=%d nred=%d", n, nred);
+  if (n > DPNEGF_LEAD_NMAX || nred > DPNEGF_LEAD_NMAX)
+    return fail(DPNEGF_ERR_UNSUPPORTED, "lead size %d / coupling rows %d exceed %d", n, nred,
+                DPNEGF_LEAD_NMAX);
+  if (mode < 0 || mode > 2) return fail(DPNEGF_ERR_ARG, "mode must be 0, 1 or 2");
+  if (mode != 0 && nred != n) return fail(DPNEGF_ERR_ARG, "modes 1/2 need nred == n");
+  if (P < 0) return fail(DPNEGF_ERR_ARG, "P must be >= 0");
+  if (P == 0) return 0;
+  if (!lead || !e || !eps_ws || !gs || !sigma) return fail(DPNEGF_ERR_ARG, "null device pointer");
+  if (nsys < 1) return fail(DPNEGF_ERR_ARG, "nsys must be >= 1");
+  if (nsys > 1 && !sys_idx) return fail(DPNEGF_ERR_ARG, "sys_idx required when nsys > 1");
+  const int64_t need = 4 * (int64_t)n * n + 2 * (int64_t)nred * n;
+  if (nsys > 1 && lead_stride < need) return fail(DPNEGF_ERR_ARG, "lead_stride too small");
+  negf::LeadArgs a;
+  a.n = n; a.nred = nred; a.mode = mode; a.lead = (const cx<double>*)lead;
+  a.lead_stride = lead_stride; a.sys_idx = sys_idx; a.P = P; a.e = (const cx<double>*)e;
+  a.eta = eta; a.eps = (cx<double>*)eps_ws; a.gs = (cx<double>*)gs; a.sigma = (cx<double>*)sigma;
+  a.iters = iters; a.status = status; a.resid = resid; a.stop = stop;
+  hipError_t err = negf::launch_lead_self_energy(a, ortho != 0, (hipStream_t)stream);
+  if (err != hipSuccess) return hip_fail(err, "lead_self_energy launch");
+  return 0;
+}
+
+int64_t dpnegf_rgf_full_workspace_elems(int K, const int32_t* h_bsz, int B) {
+  int nmax = 0;
+  if (!h_bsz || K < 1 || B < 0) return -1;
+  for (int q = 0; q < K; ++q) nmax = h_bsz[q] > nmax ? h_bsz[q] : nmax;
+  return 5 * (int64_t)B * nmax * nmax;
+}
+
+int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena, int B,
+                    const void* z, const void* sigL, int nL, const void* sigR, int nR, int ortho,
+                    void* gl, void* grd, void* grl, void* gru, void* lc, void* g_trans,
+                    const void* s_in, void* gin, void* gnd, void* gnl, void* gnu,
+                    const void* s_out, void* gip, void* gpd, void* gpl, void* gpu, void* work,
+                    int32_t* status) {
+  int nmax = 0;
+  if (int r = check_blocks(K, h_bsz, DPNEGF_FULL_NMAX, &nmax)) return r;
+  if (B < 0) return fail(DPNEGF_ERR_ARG, "B must be >= 0");
+  if (B == 0) return 0;
+  if (B > 65535) return fail(DPNEGF_ERR_ARG, "B=%d exceeds 65535 per call", B);
+  if (!arena || !z || !gl || !grd || !g_trans || !work)
+    return fail(DPNEGF_ERR_ARG, "null device pointer");
+  if (K > 1 && (!grl || !gru)) return fail(DPNEGF_ERR_ARG, "grl/gru required for K > 1");
+  if ((sigL && (nL < 1 || nL > h_bsz[0])) || (sigR && (nR < 1 || nR > h_bsz[K - 1])))
+    return fail(DPNEGF_ERR_ARG, "self-energy larger than its corner block");
+  if (s_in && (!gin || !gnd || (K > 1 && (!gnl || !gnu))))
+    return fail(DPNEGF_ERR_ARG, "lesser outputs missing");
+  if (s_out && (!gip || !gpd || (K > 1 && (!gpl || !gpu))))
+    return fail(DPNEGF_ERR_ARG, "greater outputs missing");
+  negf::RgfFullArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.K = K; a.B = B; a.nmax = nmax; a.ortho = ortho; a.bsz = h_bsz;
+  a.arena = (const cx<double>*)arena; a.z = (const cx<double>*)z;
+  a.sigL = (const cx<double>*)sigL; a.nL = nL; a.sigR = (const cx<double>*)sigR; a.nR = nR;
+  a.gl = (cx<double>*)gl; a.grd = (cx<double>*)grd; a.grl = (cx<double>*)grl;
+  a.gru = (cx<double>*)gru; a.lc = (cx<double>*)lc; a.g_trans = (cx<double>*)g_trans;
+  a.s_in = (const cx<double>*)s_in; a.gin = (cx<double>*)gin; a.gnd = (cx<double>*)gnd;
+  a.gnl = (cx<double>*)gnl; a.gnu = (cx<double>*)gnu;
+  a.s_out = (const cx<double>*)s_out; a.gip = (cx<double>*)gip; a.gpd = (cx<double>*)gpd;
+  a.gpl = (cx<double>*)gpl; a.gpu = (cx<double>*)gpu;
+  a.work = (cx<double>*)work; a.status = status;
+  hipError_t e = negf::rgf_full(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "rgf_full");
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// Shared device-side helpers for the NEGF kernels (gfx950, wave64).
+//
+// Complex numbers are stored interleaved (re, im), the storage of torch's
+// complex128 / complex64, so tensors are passed zero-copy.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace negf {
+
+constexpr int WAVE = 64;
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cx {
+  R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ cx<R> mk(R a, R b) { return cx<R>{a, b}; }
+template <typename R>
+__device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename R>
+__device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename R>
+__device__ __forceinline__ cx<R> operator-(cx<R> a) { return {-a.re, -a.im}; }
+template <typename R>
+__device__ __forceinline__ cx<R> operator*(cx<R> a, cx<R> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename R>
+__device__ __forceinline__ cx<R> conj(cx<R> a) { return {a.re, -a.im}; }
+template <typename R>
+__device__ __forceinline__ R abs1(cx<R> a) { return fabs(a.re) + fabs(a.im); }
+template <typename R>
+__device__ __forceinline__ R cabs(cx<R> a) { return sqrt(a.re * a.re + a.im * a.im); }
+// 1/d by Smith's scaling (avoids overflow of |d|^2 for large/small d)
+template <typename R>
+__device__ __forceinline__ cx<R> recip(cx<R> d) {
+  if (fabs(d.re) >= fabs(d.im)) {
+    R r = d.im / d.re, den = d.re + d.im * r;
+    return {R(1) / den, -r / den};
+  } else {
+    R r = d.re / d.im, den = d.re * r + d.im;
+    return {r / den, R(-1) / den};
+  }
+}
+template <typename R, typename S>
+__device__ __forceinline__ cx<R> cvt(cx<S> a) { return {R(a.re), R(a.im)}; }
+
+// ---- matrix-core wrappers: D(16x16) += A(16x4) * B(4x16) ------------------------
+// Operand layout (both precisions): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; the accumulator holds D[4*(l/16)+r][l%16] in element r.
+template <typename R> struct mfma;
+template <> struct mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// ---- wave / workgroup reductions -------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+
+// Sum over the workgroup; `red` is LDS scratch of >= nwaves entries.  All
+// threads get the result.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* red) {
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const int nw = (blockDim.x + WAVE - 1) / WAVE;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  return s;
+}
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T* red) {
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const int nw = (blockDim.x + WAVE - 1) / WAVE;
+  v = wave_max(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = red[0];
+  for (int i = 1; i < nw; ++i) s = fmax(s, red[i]);
+  return s;
+}
+
+}  // namespace negf

@@ -1,0 +1,53 @@
+// Argument blocks of the batched block-operation path (full RGF outputs).
+#pragma once
+#include "common.hpp"
+
+namespace negf {
+
+enum { OP_N = 0, OP_H = 1, OP_T = 2, OP_C = 3 };
+constexpr int GJ_NMAX_LARGE = 168;
+
+// Operand value (i, k) = op(h[b] - z_b s[b]) with s optional, row-major ld.
+struct Opnd {
+  const cx<double>* h;
+  const cx<double>* s;
+  int64_t bstride;
+  int64_t sstride;
+  int ld;
+  int op;
+};
+
+struct BGemmArgs {
+  int m, n, k, B;
+  Opnd A, B_;
+  cx<double> alpha, beta;
+  cx<double>* C;
+  int64_t cstride;
+  int ldc;
+  const cx<double>* z;
+};
+
+struct RgfFullArgs {
+  int K, B, nmax, ortho;
+  const int32_t* bsz;          // host [K]
+  const cx<double>* arena;     // device, one system
+  const cx<double>* z;         // device [B]
+  const cx<double>* sigL; int nL;
+  const cx<double>* sigR; int nR;
+  cx<double>* gl;              // [sum_q B n_q^2]          left-connected g_q
+  cx<double>* grd;             // [sum_q B n_q^2]
+  cx<double>* grl;             // [sum_q B n_q+1 n_q]
+  cx<double>* gru;             // [sum_q B n_q n_q+1]
+  cx<double>* lc;              // [sum_q B n_q n_K-1] or null
+  cx<double>* g_trans;         // [B n_0 n_K-1]
+  const cx<double>* s_in;      // lesser source, [sum_q B n_q^2] or null
+  cx<double>* gin; cx<double>* gnd; cx<double>* gnl; cx<double>* gnu;
+  const cx<double>* s_out;     // greater source or null
+  cx<double>* gip; cx<double>* gpd; cx<double>* gpl; cx<double>* gpu;
+  cx<double>* work;            // 5 * B * nmax^2
+  int32_t* status;             // [B] or null
+};
+
+hipError_t rgf_full(const RgfFullArgs& a, hipStream_t st);
+
+}  // namespace negf

@@ -1,0 +1,74 @@
+// Kernel argument blocks and launchers shared by the C ABI and the kernels.
+#pragma once
+#include "wg_linalg.hpp"
+
+namespace negf {
+
+enum : int32_t {
+  STATUS_SINGULAR = 1,       // exactly-zero pivot in a block inverse
+  STATUS_LS_NOCONV = 2,      // Lopez-Sancho: > 100 iterations
+  STATUS_LS_RESID_FAIL = 4,  // Lopez-Sancho: Dyson residual >= 3e-5
+  STATUS_LS_RESID_WARN = 8,  // Lopez-Sancho: Dyson residual > 1e-8
+  STATUS_REFINE_FALLBACK = 16,  // mixed precision: refinement did not converge
+};
+
+// Largest per-thread row count of wg_gj_inverse over all n <= nmax.
+constexpr int gj_maxr(int nmax, int nt) {
+  int m = 1;
+  for (int n = 1; n <= nmax; ++n) {
+    const int G = nt / n;
+    const int r = (n + G - 1) / G;
+    if (r > m) m = r;
+  }
+  return m;
+}
+
+// Gamma = i (Sigma - Sigma^dagger), element (i, j) of an n x n Sigma.
+__device__ __forceinline__ cx<double> gamma_of(const cx<double>* s, int n, int i, int j) {
+  const cx<double> a = s[i * n + j], b = s[j * n + i];
+  const double wr = a.re - b.re, wi = a.im + b.im;  // a - conj(b)
+  return cx<double>{-wi, wr};
+}
+
+struct RgfArgs {
+  int K;
+  const int32_t* bsz;        // device [K]
+  const cx<double>* arena;   // device, nsys systems
+  int64_t sys_stride;        // elements between systems
+  const int32_t* sys_idx;    // device [P] or null (all system 0)
+  int P;
+  const cx<double>* z;       // device [P]: E + E_ref + i eta
+  const cx<double>* sigL;    // device [P, nL, nL]
+  int nL;
+  const cx<double>* sigR;    // device [P, nR, nR]
+  int nR;
+  double* T;                 // device [P]
+  int32_t* status;           // device [P] or null
+};
+
+template <typename R>
+hipError_t launch_rgf_transmission(const RgfArgs& a, int nmax, bool ortho, hipStream_t st);
+hipError_t launch_rgf_transmission_mixed(const RgfArgs& a, int nmax, bool ortho, hipStream_t st);
+
+struct LeadArgs {
+  int n;                     // principal-layer size
+  int nred;                  // rows of the device-lead coupling (Sigma is nred x nred)
+  int mode;                  // 0: Sigma = tau g tau', 1: (eS - H) - g^{-1}, 2: g^{-1}
+  const cx<double>* lead;    // per system: H, h01, S, s01 [n x n], hDL, sDL [nred x n]
+  int64_t lead_stride;
+  const int32_t* sys_idx;    // [P] or null
+  int P;
+  const cx<double>* e;       // [P] E + E_ref (no eta)
+  double eta;                // eta_lead
+  cx<double>* eps;           // workspace [P, n, n]
+  cx<double>* gs;            // out [P, n, n]
+  cx<double>* sigma;         // out [P, nred, nred]
+  int32_t* iters;            // [P] or null
+  int32_t* status;           // [P] or null
+  double* resid;             // [P] or null
+  double stop;               // decimation stop threshold on max(|alpha| + |beta|)
+};
+
+hipError_t launch_lead_self_energy(const LeadArgs& a, bool ortho, hipStream_t st);
+
+}  // namespace negf

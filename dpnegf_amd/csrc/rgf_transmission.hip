@@ -1,0 +1,175 @@
+// Fused transmission sweep: one workgroup per (E,k) point, blocks in LDS.
+//
+// Replaces the forward loop of recursive_gf_cal (dpnegf/negf/recursive_green_cal.py
+// :198-207), the g_trans product of its backward loop (:223-228) and the trace of
+// DeviceProperty._cal_tc_ (dpnegf/negf/device_property.py:383-404).  Instead of
+// storing every g_q and walking back, the sweep carries P_q = G^L_{0,q}:
+//
+//   g_0 = (-M_0)^{-1},  P_0 = g_0
+//   Y   = g_q U_q,      P  <- P U_q                     (one stacked product)
+//   g_{q+1} = (-M_{q+1} - L_q Y)^{-1}
+//   P_{q+1} = P U_q g_{q+1}
+//
+// so P_{K-1} = g_0 U_0 g_1 ... U_{K-2} g_{K-1} = G_{0,K-1}, the product the
+// reference builds right-to-left.  Then T = Re Tr[Gamma_L P Gamma_R P^dagger],
+// Gamma = i(Sigma - Sigma^dagger) formed on the fly from the self-energies.
+//
+// Arena layout of one system (complex128, row-major), for q = 0..K-1:
+//   hd_q [n_q x n_q], sd_q [n_q x n_q], and for q < K-1:
+//   hu_q [n_q x n_q+1], su_q, hl_q [n_q+1 x n_q], sl_q.
+#include "rgf_kernels.hpp"
+
+namespace negf {
+
+template <typename R, int NMAX, bool ORTHO>
+__global__ __launch_bounds__(256) void rgf_transmission_kernel(RgfArgs a) {
+  constexpr int LD = NMAX + 1;
+  constexpr int NT = (NMAX / 16) * (NMAX / 16);
+  constexpr int MAXT_STACK = (2 * NT + 3) / 4;
+  constexpr int MAXT_SQ = (NT + 3) / 4;
+  constexpr int MAXR = gj_maxr(NMAX, 256);
+  __shared__ cx<R> PG[2 * NMAX * LD];
+  __shared__ GJScratch<R, NMAX> gjs;
+  __shared__ double red[8];
+
+  const int pt = blockIdx.x;
+  if (pt >= a.P) return;  // uniform per workgroup
+  const int sys = a.sys_idx ? a.sys_idx[pt] : 0;
+  const cx<double>* base = a.arena + (int64_t)sys * a.sys_stride;
+  const cx<double> zd = a.z[pt];
+  const int K = a.K;
+  const int n0 = a.bsz[0];
+  const int nlast = a.bsz[K - 1];
+  const cx<double>* sL = a.sigL + (int64_t)pt * a.nL * a.nL;
+  const cx<double>* sR = a.sigR + (int64_t)pt * a.nR * a.nR;
+  const int nL = a.nL, nR = a.nR;
+
+  cx<R>* P = PG;
+  cx<R>* g = PG + n0 * LD;
+
+  // M(i,k) = H - z S (+ Sigma corners), in double before rounding to R
+  auto Mval = [&](const cx<double>* h, const cx<double>* s, int nb, int i, int k, bool left,
+                  bool right) -> cx<double> {
+    cx<double> v = h[i * nb + k];
+    if (ORTHO) {
+      if (i == k) v = v - zd;
+    } else {
+      v = v - zd * s[i * nb + k];
+    }
+    if (left && i < nL && k < nL) v = v + sL[i * nL + k];
+    if (right) {
+      const int o = nb - nR;
+      if (i >= o && k >= o) v = v + sR[(i - o) * nR + (k - o)];
+    }
+    return v;
+  };
+
+  const cx<double>* hd0 = base;
+  const cx<double>* sd0 = base + n0 * n0;
+  int sing = wg_gj_inverse<R, 256, NMAX, MAXR>(
+      n0, [&](int i, int k) { return cvt<R>(-Mval(hd0, sd0, n0, i, k, true, K == 1)); },
+      [&](int i, int k, cx<R> v) { g[i * LD + k] = v; }, &gjs);
+  __syncthreads();
+  for (int e = threadIdx.x; e < n0 * n0; e += 256) {
+    const int i = e / n0, k = e % n0;
+    P[i * LD + k] = g[i * LD + k];
+  }
+  __syncthreads();
+
+  int64_t off = 0;
+  for (int q = 0; q < K - 1; ++q) {
+    const int nq = a.bsz[q], n1 = a.bsz[q + 1];
+    const cx<double>* hu = base + off + 2 * (int64_t)nq * nq;
+    const cx<double>* su = hu + (int64_t)nq * n1;
+    const cx<double>* hl = su + (int64_t)nq * n1;
+    const cx<double>* sl = hl + (int64_t)n1 * nq;
+    const int64_t off1 = off + 2 * (int64_t)nq * nq + 4 * (int64_t)nq * n1;
+    const cx<double>* hd1 = base + off1;
+    const cx<double>* sd1 = hd1 + (int64_t)n1 * n1;
+    const bool last = (q + 1 == K - 1);
+
+    // [P; g] <- [P; g] (U_q - z SU_q)
+    wg_gemm<R, 4, MAXT_STACK>(
+        n0 + nq, n1, nq, [&](int i, int k) { return PG[i * LD + k]; },
+        [&](int k, int j) {
+          cx<double> u = hu[k * n1 + j];
+          if (!ORTHO) u = u - zd * su[k * n1 + j];
+          return cvt<R>(u);
+        },
+        [&](int i, int j, cx<R> v) { PG[i * LD + j] = v; });
+    __syncthreads();
+    // g <- -M_{q+1} - (L_q - z SL_q) Y
+    wg_gemm<R, 4, MAXT_SQ>(
+        n1, n1, nq,
+        [&](int i, int k) {
+          cx<double> l = hl[i * nq + k];
+          if (!ORTHO) l = l - zd * sl[i * nq + k];
+          return cvt<R>(l);
+        },
+        [&](int k, int j) { return g[k * LD + j]; },
+        [&](int i, int j, cx<R> v) {
+          g[i * LD + j] = cvt<R>(-(cvt<double>(v) + Mval(hd1, sd1, n1, i, j, false, last)));
+        });
+    __syncthreads();
+    sing |= wg_gj_inverse<R, 256, NMAX, MAXR>(
+        n1, [&](int i, int k) { return g[i * LD + k]; },
+        [&](int i, int k, cx<R> v) { g[i * LD + k] = v; }, &gjs);
+    __syncthreads();
+    // P <- P g_{q+1}
+    wg_gemm<R, 4, MAXT_SQ>(
+        n0, n1, n1, [&](int i, int k) { return P[i * LD + k]; },
+        [&](int k, int j) { return g[k * LD + j]; },
+        [&](int i, int j, cx<R> v) { P[i * LD + j] = v; });
+    __syncthreads();
+    off = off1;
+  }
+
+  // T = Re sum_{a,b} GammaL[a][b] (Q GammaR Q^dagger)[b][a],  Q = P[0:nL, c0:c0+nR]
+  const int c0 = nlast - nR;
+  cx<R>* V = g;
+  wg_gemm<R, 4, MAXT_SQ>(
+      nL, nR, nR, [&](int i, int k) { return P[i * LD + c0 + k]; },
+      [&](int k, int j) { return cvt<R>(gamma_of(sR, nR, k, j)); },
+      [&](int i, int j, cx<R> v) { V[i * LD + j] = v; });
+  __syncthreads();
+  double acc = 0.0;
+  wg_gemm<R, 4, MAXT_SQ>(
+      nL, nL, nR, [&](int i, int k) { return V[i * LD + k]; },
+      [&](int k, int j) { return conj(P[j * LD + c0 + k]); },
+      [&](int i, int j, cx<R> x) {
+        const cx<double> gl = gamma_of(sL, nL, j, i);
+        acc += gl.re * (double)x.re - gl.im * (double)x.im;
+      });
+  const double T = block_sum(acc, red);
+  if (threadIdx.x == 0) {
+    a.T[pt] = T;
+    if (a.status) a.status[pt] = sing ? STATUS_SINGULAR : 0;
+  }
+}
+
+template <typename R, int NMAX>
+static hipError_t launch_nmax(const RgfArgs& a, bool ortho, hipStream_t st) {
+  dim3 grid(a.P), block(256);
+  if (ortho)
+    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, true>), grid, block, 0, st, a);
+  else
+    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, false>), grid, block, 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_rgf_transmission(const RgfArgs& a, int nmax, bool ortho, hipStream_t st) {
+  if (nmax <= 16) return launch_nmax<R, 16>(a, ortho, st);
+  if (nmax <= 32) return launch_nmax<R, 32>(a, ortho, st);
+  if (nmax <= 48) return launch_nmax<R, 48>(a, ortho, st);
+  if (nmax <= 64) return launch_nmax<R, 64>(a, ortho, st);
+  return hipErrorInvalidValue;
+}
+
+template hipError_t launch_rgf_transmission<double>(const RgfArgs&, int, bool, hipStream_t);
+
+hipError_t launch_rgf_transmission_mixed(const RgfArgs& a, int nmax, bool ortho, hipStream_t st) {
+  return launch_rgf_transmission<float>(a, nmax, ortho, st);
+}
+
+}  // namespace negf

@@ -1,0 +1,263 @@
+"""Device observables on the GPU (``dpnegf.negf.device_property`` interface).
+
+* :class:`DeviceProperty` -- ``cal_green_function`` (``device_property.py:137-296``),
+  ``_cal_tc_`` (``:374-406``), ``_cal_dos_`` (``:408-439``), ``_cal_ldos_``
+  (``:441-470``) and the Green's-function properties, backed by
+  :func:`recursive_gf`.
+* :func:`transmission_pipeline` -- new: T(E, k) for a whole energy grid and
+  k-mesh in two launches (all lead self-energies, then the fused sweep),
+  replacing the runner's chunk loop (``runner/NEGF.py:750-816``) for T.
+"""
+import math
+import os
+
+import numpy as np
+import torch
+
+from .. import _native as nat
+from ._btd import C128
+from .recursive_green_cal import BTDSystems, recursive_gf, rgf_transmission
+from .surface_green import self_energy_batched
+
+_EV2J = 1.6021766208e-19
+_BOLTZMANN = 1.380649e-23
+
+
+def kBT_of(e_T):
+    """kB T in eV with the reference's constants (``device_property.py:105``)."""
+    return _BOLTZMANN * e_T / _EV2J
+
+
+def _build_s_in_batched(hd, seinL, seinR, idx0, idy0, idx1, idy1):
+    """``device_property.py:21-31``."""
+    B = seinL.shape[0]
+    s_in = [torch.zeros((B,) + tuple(blk.shape), dtype=C128, device=blk.device) for blk in hd]
+    s_in[0][:, :idx0, :idy0] = s_in[0][:, :idx0, :idy0] + seinL[:, :idx0, :idy0]
+    s_in[-1][:, -idx1:, -idy1:] = s_in[-1][:, -idx1:, -idy1:] + seinR[:, -idx1:, -idy1:]
+    return s_in
+
+
+class DeviceProperty(object):
+    """GPU-backed stand-in for the reference's ``DeviceProperty``.
+
+    The Hamiltonian object must provide ``get_hs_device(kpoint, V,
+    block_tridiagonal)`` returning ``(hd, sd, hl, su, sl, hu)`` and
+    ``device_norbs`` (as in the reference); leads provide ``se``, ``gamma`` and
+    ``fermi_dirac`` (``LeadProperty``).
+    """
+
+    def __init__(self, hamiltonian, structure, results_path, e_T=300, efermi=None, chemiPot=None,
+                 E_ref=None, rgf_device="cuda"):
+        self.greenfuncs = 0
+        self.hamiltonian = hamiltonian
+        self.structure = structure
+        self.results_path = results_path
+        self.cdtype = C128
+        self.rgf_device = torch.device(rgf_device) if isinstance(rgf_device, str) else rgf_device
+        self.kBT = kBT_of(e_T)
+        self.e_T = e_T
+        self.chemiPot = chemiPot
+        self.efermi = efermi
+        self.E_ref = efermi if E_ref is None else E_ref
+        self.kpoint = None
+        self.newK_flag = None
+        self.newV_flag = None
+
+    def set_leadLR(self, lead_L, lead_R):
+        self.lead_L = lead_L
+        self.lead_R = lead_R
+
+    def cal_green_function(self, energy, kpoint, eta_device=0., block_tridiagonal=True, Vbias=None,
+                           HS_inmem=True, need_lesser=False, need_greater=False, need_gr_lc=False):
+        """``device_property.py:137-296`` on the GPU."""
+        assert len(np.array(kpoint).reshape(-1)) == 3
+        energy = torch.as_tensor(energy, dtype=C128).to(self.rgf_device)
+        if energy.ndim == 0:
+            energy = energy.reshape(1)
+        assert energy.ndim == 1
+        B = energy.shape[0]
+        batched = B > 1
+        self.block_tridiagonal = block_tridiagonal
+        kp = torch.tensor(kpoint)
+        if self.kpoint is None or abs(self.kpoint - kp).sum() > 1e-5:
+            self.kpoint = kp
+            self.newK_flag = True
+        else:
+            self.newK_flag = False
+        self.oldV = getattr(self, "V", None)
+        if Vbias is None:
+            pot = os.path.join(self.results_path, "POTENTIAL.pth") if self.results_path else ""
+            if pot and os.path.exists(pot):
+                self.V = torch.load(pot, weights_only=True)
+            else:
+                self.V = torch.tensor(0.)
+        else:
+            self.V = Vbias
+        assert torch.is_tensor(self.V)
+        self.V = self.V.to(self.rgf_device)
+        if self.oldV is not None:
+            self.newV_flag = bool(torch.abs(self.V - self.oldV.to(self.rgf_device)).sum() > 1e-5)
+        else:
+            self.newV_flag = True
+        if (not (hasattr(self, "hd") and hasattr(self, "sd"))) or self.newK_flag or self.newV_flag:
+            hs = self.hamiltonian.get_hs_device(self.kpoint, self.V, block_tridiagonal)
+            self.hd, self.sd, self.hl, self.su, self.sl, self.hu = [
+                [b.to(self.rgf_device) for b in blocks] for blocks in hs]
+
+        tags = ["g_trans", "gr_lc", "grd", "grl", "gru", "gr_left", "gnd", "gnl", "gnu",
+                "gin_left", "gpd", "gpl", "gpu", "gip_left"]
+        seL = self.lead_L.se.to(self.rgf_device)
+        seR = self.lead_R.se.to(self.rgf_device)
+        if batched:
+            assert seL.ndim == 3 and seR.ndim == 3
+        else:
+            assert seL.ndim == 2 and seR.ndim == 2
+        s01, s02 = self.hd[0].shape
+        se01, se02 = seL.shape[-2], seL.shape[-1]
+        s11, s12 = self.hd[-1].shape
+        se11, se12 = seR.shape[-2], seR.shape[-1]
+        idx0, idy0 = min(s01, se01), min(s02, se02)
+        idx1, idy1 = min(s11, se11), min(s12, se12)
+        if block_tridiagonal:
+            if se01 > s01 or se02 > s02:
+                raise ValueError("Left Lead Self Energy size is larger than the first Hamiltonian Block.")
+            if se11 > s11 or se12 > s12:
+                raise ValueError("Right Lead Self Energy size is larger than the last Hamiltonian Block.")
+        if need_lesser:
+            if batched:
+                fL = self.lead_L.fermi_dirac(energy + self.E_ref).reshape(B, 1, 1)
+                fR = self.lead_R.fermi_dirac(energy + self.E_ref).reshape(B, 1, 1)
+                seinL = 1j * (seL - seL.mH) * fL
+                seinR = 1j * (seR - seR.mH) * fR
+            else:
+                seinL = (1j * (seL - seL.conj().T) * self.lead_L.fermi_dirac(energy + self.E_ref).reshape(-1)).unsqueeze(0)
+                seinR = (1j * (seR - seR.conj().T) * self.lead_R.fermi_dirac(energy + self.E_ref).reshape(-1)).unsqueeze(0)
+            s_in = _build_s_in_batched(self.hd, seinL, seinR, idx0, idy0, idx1, idy1)
+            if not batched:
+                s_in = [t[0] for t in s_in]
+        else:
+            s_in = 0
+        keep_gr_left = bool(need_lesser or need_greater)
+        ans = recursive_gf(energy, hl=self.hl, hd=self.hd, hu=self.hu, sd=self.sd, su=self.su,
+                           sl=self.sl, left_se=seL, right_se=seR, seP=None, s_in=s_in, s_out=None,
+                           eta=eta_device, E_ref=self.E_ref, need_lesser=need_lesser,
+                           need_greater=need_greater, need_gr_lc=need_gr_lc,
+                           keep_gr_left=keep_gr_left)
+        self.greenfuncs = {t: a for t, a in zip(tags, ans)}
+        if not HS_inmem:
+            del self.hd, self.sd, self.hl, self.su, self.sl, self.hu
+
+    def release_greenfuncs(self):
+        self.greenfuncs = 0
+        torch.cuda.empty_cache()
+
+    def _cal_tc_(self):
+        """``device_property.py:374-406``."""
+        g_trans = self.g_trans
+        batched = g_trans.ndim == 3
+        tx, ty = g_trans.shape[-2], g_trans.shape[-1]
+        gL_full = self.lead_L.gamma.to(self.rgf_device)
+        gR_full = self.lead_R.gamma.to(self.rgf_device)
+        x0, x1 = min(gL_full.shape[-2], tx), min(gR_full.shape[-2], ty)
+        gl_shape = (g_trans.shape[0], tx, tx) if batched else (tx, tx)
+        gr_shape = (g_trans.shape[0], ty, ty) if batched else (ty, ty)
+        gL = torch.zeros(gl_shape, dtype=C128, device=self.rgf_device)
+        gR = torch.zeros(gr_shape, dtype=C128, device=self.rgf_device)
+        gL[..., :x0, :x0] += gL_full[..., :x0, :x0]
+        gR[..., -x1:, -x1:] += gR_full[..., -x1:, -x1:]
+        return (gL @ g_trans @ gR @ g_trans.mH).diagonal(dim1=-2, dim2=-1).real.sum(-1)
+
+    def _spectral_blocks(self):
+        out = []
+        K = len(self.grd)
+        for jj in range(K):
+            if not self.block_tridiagonal or len(self.gru) == 0:
+                t = self.grd[jj] @ self.sd[jj]
+            elif jj == 0:
+                t = self.grd[jj] @ self.sd[jj] + self.gru[jj] @ self.sl[jj]
+            elif jj == K - 1:
+                t = self.grd[jj] @ self.sd[jj] + self.grl[jj - 1] @ self.su[jj - 1]
+            else:
+                t = (self.grd[jj] @ self.sd[jj] + self.grl[jj - 1] @ self.su[jj - 1]
+                     + self.gru[jj] @ self.sl[jj])
+            out.append(t)
+        return out
+
+    def _cal_dos_(self):
+        """``device_property.py:408-439``."""
+        dos = 0
+        for t in self._spectral_blocks():
+            dos = dos - t.imag.diagonal(dim1=-2, dim2=-1).sum(-1) / math.pi
+        return dos * 2
+
+    def _cal_ldos_(self):
+        """``device_property.py:441-470``."""
+        ldos = torch.cat([-t.imag.diagonal(dim1=-2, dim2=-1) / math.pi
+                          for t in self._spectral_blocks()], dim=-1).contiguous()
+        acc = np.cumsum([0] + list(self.norbs_per_atom))
+        ldos = torch.stack([ldos[..., acc[i]:acc[i + 1]].sum(-1) for i in range(len(acc) - 1)],
+                           dim=-1)
+        return ldos * 2
+
+    @property
+    def dos(self):
+        return self._cal_dos_()
+
+    @property
+    def ldos(self):
+        return self._cal_ldos_()
+
+    @property
+    def tc(self):
+        return self._cal_tc_()
+
+    @property
+    def norbs_per_atom(self):
+        return self.hamiltonian.device_norbs
+
+    def __getattr__(self, name):
+        tags = ("g_trans", "gr_lc", "grd", "grl", "gru", "gr_left", "gnd", "gnl", "gnu",
+                "gin_left", "gpd", "gpl", "gpu", "gip_left")
+        if name in tags:
+            return self.__dict__["greenfuncs"][name]
+        raise AttributeError(name)
+
+
+def _lead_tuple(lead, dev):
+    return tuple(x.to(dev) if x is not None else None for x in lead)
+
+
+def transmission_pipeline(systems, energies, eta_lead, eta_device, E_ref=0.0, precision="fp64",
+                          return_status=False):
+    """T(E, k) for ``systems`` (one :class:`~dpnegf_amd.workloads.BTDSystem` or a
+    list, one per k-point, identical block sizes) on the energy grid
+    ``energies`` (relative to ``E_ref``).  Returns [nE] (or [nk, nE])."""
+    single = not isinstance(systems, (list, tuple))
+    sys_list = [systems] if single else list(systems)
+    dev = sys_list[0].hd[0].device
+    nat.require_device(sys_list[0].hd[0])
+    E = torch.as_tensor(energies).to(device=dev, dtype=C128).reshape(-1)
+    nk, nE = len(sys_list), E.shape[0]
+    e_all = E.repeat(nk)
+    sys_idx = torch.arange(nk, dtype=torch.int32, device=dev).repeat_interleave(nE).contiguous()
+    sig = []
+    for side in ("lead_L", "lead_R"):
+        leads = [_lead_tuple(getattr(s, side), dev) for s in sys_list]
+        res = self_energy_batched(leads, e_all, eta_lead, sys_idx=sys_idx if nk > 1 else None,
+                                  E_ref=E_ref)
+        sig.append(res.sigma)
+    n0, nK = sys_list[0].sizes[0], sys_list[0].sizes[-1]
+    sL = sig[0][:, :min(n0, sig[0].shape[-1]), :min(n0, sig[0].shape[-1])].contiguous()
+    rR = min(nK, sig[1].shape[-1])
+    sR = sig[1][:, -rR:, -rR:].contiguous()
+    bs = BTDSystems([(s.hd, s.hu, s.hl, s.sd, s.su, s.sl) for s in sys_list], dev)
+    z = (e_all + E_ref + 1j * eta_device).contiguous()
+    status = torch.zeros(nk * nE, dtype=torch.int32, device=dev)
+    T = rgf_transmission(bs, z, sL, sR, sys_idx=sys_idx if nk > 1 else None,
+                         precision=precision, status=status)
+    T = T.view(nk, nE)
+    if single:
+        T = T[0]
+    if return_status:
+        return T, status
+    return T

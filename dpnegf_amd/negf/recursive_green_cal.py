@@ -1,0 +1,185 @@
+"""Recursive Green's function on the GPU, ``dpnegf.negf.recursive_green_cal`` interface.
+
+* :func:`recursive_gf` -- same signature, defaults and 14-tuple as the
+  reference wrapper (``dpnegf/negf/recursive_green_cal.py:324-477``); every
+  block is computed by ``dpnegf_rgf_full`` (batched block operations, all
+  quirks of ``recursive_gf_cal`` kept).
+* :func:`rgf_transmission` -- new entry: T(E, k) only, one fused kernel
+  launch over every (E, k) point (``dpnegf_rgf_transmission``).
+
+Inputs must be CUDA (HIP) tensors; there is no CPU path.
+"""
+import torch
+
+from .. import _native as nat
+from ._btd import (C128, block_sizes, corner_self_energy, is_orthogonal, pack_arena,
+                   pack_batched_blocks, split_blocks)
+
+_MAX_B = 65535
+
+
+def _squeeze_ans(ans):
+    """``_squeeze_ans`` (``recursive_green_cal.py:480-491``)."""
+    def sq(x):
+        if x is None:
+            return None
+        if torch.is_tensor(x):
+            return x.squeeze(0)
+        if isinstance(x, list):
+            return [t.squeeze(0) if torch.is_tensor(t) else t for t in x]
+        return x
+    return tuple(sq(x) for x in ans)
+
+
+def _energies(energy, E_ref, left_se, right_se, device):
+    e = energy + E_ref
+    if not torch.is_tensor(e):
+        e = torch.as_tensor(e, dtype=C128)
+    probe = left_se if isinstance(left_se, torch.Tensor) else right_se
+    squeezed = isinstance(probe, torch.Tensor) and probe.ndim == 2
+    if e.ndim == 0:
+        e = e.reshape(1)
+    return e.to(device=device, dtype=C128), squeezed
+
+
+def recursive_gf(energy, hl, hd, hu, sd, su, sl, left_se, right_se, seP=None, E_ref=0.0, s_in=0,
+                 s_out=0, eta=1e-5, need_lesser=False, need_greater=False, need_gr_lc=False,
+                 keep_gr_left=True):
+    """Retarded (and optionally lesser / greater) Green's function blocks.
+
+    Returns ``(g_trans, gr_lc, grd, grl, gru, gr_left, gnd, gnl, gnu, gin_left,
+    gpd, gpl, gpu, gip_left)`` exactly as ``recursive_green_cal.py:299-321``;
+    the leading energy dimension is squeezed for scalar callers (2-D
+    self-energies), ``grl == gru == []`` for a single block.
+    """
+    device = hd[0].device
+    nat.require_device(*hd)
+    e, squeezed = _energies(energy, E_ref, left_se, right_se, device)
+    B = e.shape[0]
+    z = (e + 1j * eta).contiguous()
+    if need_lesser:
+        assert isinstance(s_in, list), "need_lesser=True requires s_in to be a list of coupling matrices"
+    if need_greater:
+        assert isinstance(s_out, list), "need_greater=True requires s_out to be a list of coupling matrices"
+
+    hd_eff = list(hd)
+    if seP is not None:
+        for i in range(len(hd_eff)):
+            sp = seP[i]
+            if torch.is_tensor(sp):
+                if sp.ndim != 2:
+                    raise NotImplementedError("energy-dependent seP blocks are not supported")
+                hd_eff[i] = hd_eff[i] + sp.to(device)
+            elif sp != 0:
+                hd_eff[i] = hd_eff[i] + sp
+    sizes = block_sizes(hd_eff)
+    K = len(sizes)
+    arena = pack_arena(hd_eff, hu, hl, sd, su, sl, device)
+    ortho = is_orthogonal(sd, su, sl)
+    sigL = sigR = None
+    nL = nR = 0
+    if isinstance(left_se, torch.Tensor):
+        sigL, nL = corner_self_energy(left_se, sizes[0], B, False, device)
+    if isinstance(right_se, torch.Tensor):
+        sigR, nR = corner_self_energy(right_se, sizes[-1], B, True, device)
+
+    sq_shapes = [(n, n) for n in sizes]
+    lo_shapes = [(sizes[q + 1], sizes[q]) for q in range(K - 1)]
+    up_shapes = [(sizes[q], sizes[q + 1]) for q in range(K - 1)]
+    lc_shapes = [(n, sizes[-1]) for n in sizes]
+    nsq = sum(n * n for n in sizes)
+    nx = sum(a * b for a, b in up_shapes)
+    nlc = sum(n * sizes[-1] for n in sizes)
+
+    def alloc(count):
+        return torch.empty(B * count, dtype=C128, device=device)
+
+    gl, grd = alloc(nsq), alloc(nsq)
+    grl, gru = alloc(max(nx, 1)), alloc(max(nx, 1))
+    lc = alloc(nlc) if need_gr_lc else None
+    g_trans = torch.empty(B, sizes[0], sizes[-1], dtype=C128, device=device)
+    lesser = greater = None
+    if need_lesser:
+        sin = pack_batched_blocks(s_in, B, device)
+        lesser = (sin, alloc(nsq), alloc(nsq), alloc(max(nx, 1)), alloc(max(nx, 1)))
+    if need_greater:
+        sout = pack_batched_blocks(s_out, B, device)
+        greater = (sout, alloc(nsq), alloc(nsq), alloc(max(nx, 1)), alloc(max(nx, 1)))
+    status = torch.zeros(B, dtype=torch.int32, device=device)
+
+    if B > _MAX_B:
+        raise ValueError(f"at most {_MAX_B} energies per recursive_gf call (got {B}); "
+                         "split the energy grid into chunks as the runner does")
+    nmax = max(sizes)
+    work = torch.empty(5 * B * nmax * nmax, dtype=C128, device=device)
+    args = [nat.stream_ptr(device), K, nat.host_i32(sizes), nat.ptr(arena), B, nat.ptr(z),
+            nat.ptr(sigL), nL, nat.ptr(sigR), nR, int(ortho),
+            nat.ptr(gl), nat.ptr(grd), nat.ptr(grl), nat.ptr(gru), nat.ptr(lc),
+            nat.ptr(g_trans)]
+    for part in (lesser, greater):
+        args += [nat.ptr(t) for t in part] if part else [nat.ptr(None)] * 5
+    args += [nat.ptr(work), nat.ptr(status)]
+    nat.call("dpnegf_rgf_full", *args)
+
+    if int(status.max().item()) & nat.STATUS_SINGULAR:
+        raise torch.linalg.LinAlgError("singular block in the recursive Green's function sweep")
+
+    grd_l = split_blocks(grd, B, sq_shapes)
+    grl_l = split_blocks(grl, B, lo_shapes)
+    gru_l = split_blocks(gru, B, up_shapes)
+    gl_l = split_blocks(gl, B, sq_shapes) if keep_gr_left else None
+    lc_l = split_blocks(lc, B, lc_shapes) if need_gr_lc else None
+    if need_gr_lc:
+        g_trans = lc_l[0]
+    ans = [g_trans, lc_l, grd_l, grl_l, gru_l, gl_l]
+    for part in (lesser, greater):
+        if part is None:
+            ans += [None] * 4
+        else:
+            _, gi, gd, gL, gU = part
+            ans += [split_blocks(gd, B, sq_shapes), split_blocks(gL, B, lo_shapes),
+                    split_blocks(gU, B, up_shapes), split_blocks(gi, B, sq_shapes)]
+    ans = tuple(ans)
+    if squeezed:
+        ans = _squeeze_ans(ans)
+    return ans
+
+
+class BTDSystems:
+    """Device-resident Hamiltonians of one or more k-points with identical
+    block sizes, packed once for repeated transmission sweeps."""
+
+    def __init__(self, systems, device=None):
+        """``systems``: list of ``(hd, hu, hl, sd, su, sl)`` block lists."""
+        if device is None:
+            device = systems[0][0][0].device
+        nat.require_device(systems[0][0][0].to(device))
+        self.device = torch.device(device)
+        self.sizes = block_sizes(systems[0][0])
+        for s in systems:
+            if block_sizes(s[0]) != self.sizes:
+                raise ValueError("all systems must share the block sizes")
+        arenas = [pack_arena(*s, self.device) for s in systems]
+        self.stride = arenas[0].numel()
+        self.arena = torch.cat(arenas)
+        self.nsys = len(systems)
+        self.ortho = all(is_orthogonal(s[3], s[4], s[5]) for s in systems)
+        self.bsz = torch.tensor(self.sizes, dtype=torch.int32, device=self.device)
+        self._h_bsz = nat.host_i32(self.sizes)
+
+
+def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", status=None, out=None):
+    """T for every point: ``z`` [P] complex (E + E_ref + i eta), ``sigL`` [P, nL, nL],
+    ``sigR`` [P, nR, nR] (already trimmed to their corners), ``sys_idx`` [P] int32
+    (k-point of each point).  Returns T [P] float64 (and fills ``status``)."""
+    if not isinstance(systems, BTDSystems):
+        raise TypeError("systems must be a BTDSystems")
+    dev = systems.device
+    P = z.shape[0]
+    T = out if out is not None else torch.empty(P, dtype=torch.float64, device=dev)
+    prec = {"fp64": nat.DPNEGF_FP64, "mixed": nat.DPNEGF_MIXED}[precision]
+    nat.call("dpnegf_rgf_transmission", nat.stream_ptr(dev), prec, len(systems.sizes),
+             systems._h_bsz, nat.ptr(systems.bsz), nat.ptr(systems.arena), systems.stride,
+             systems.nsys, nat.ptr(sys_idx), P, nat.ptr(z), nat.ptr(sigL), sigL.shape[-1],
+             nat.ptr(sigR), sigR.shape[-1], int(systems.ortho), nat.ptr(T), nat.ptr(status))
+    return T

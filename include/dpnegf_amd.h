@@ -1,0 +1,115 @@
+/* libdpnegf_amd -- C ABI of the MI355X (gfx950) NEGF hot path.
+ *
+ * Every data pointer is device (HBM) memory unless its name starts with h_.
+ * Complex numbers are interleaved (re, im) doubles, the storage of torch's
+ * complex128, so torch tensors are passed zero-copy.  `stream` is a
+ * hipStream_t (torch's current stream).  Calls are asynchronous on `stream`;
+ * they allocate no device memory (workspaces are arguments).  Return 0 on
+ * success, a negative DPNEGF_ERR_* code otherwise; dpnegf_last_error() gives
+ * the message (thread-local).
+ *
+ * Device Hamiltonian "arena" of one system (one k-point), row-major blocks in
+ * this order, q = 0..K-1:  hd_q [n_q x n_q], sd_q [n_q x n_q], then for
+ * q < K-1:  hu_q [n_q x n_{q+1}], su_q, hl_q [n_{q+1} x n_q], sl_q.
+ * These are the block lists of get_hs_device
+ * (dpnegf/negf/negf_hamiltonian_init.py:745-880).
+ */
+#ifndef DPNEGF_AMD_H
+#define DPNEGF_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPNEGF_ABI_VERSION 1
+
+#define DPNEGF_OK 0
+#define DPNEGF_ERR_ARG (-1)
+#define DPNEGF_ERR_UNSUPPORTED (-2)
+#define DPNEGF_ERR_HIP (-3)
+
+#define DPNEGF_FP64 0
+#define DPNEGF_MIXED 1
+
+/* size limits of the entry points */
+#define DPNEGF_TRANSMISSION_NMAX 64
+#define DPNEGF_LEAD_NMAX 48
+#define DPNEGF_FULL_NMAX 168
+
+/* per-point status bits */
+#define DPNEGF_STATUS_SINGULAR 1
+#define DPNEGF_STATUS_LS_NOCONV 2
+#define DPNEGF_STATUS_LS_RESID_FAIL 4
+#define DPNEGF_STATUS_LS_RESID_WARN 8
+#define DPNEGF_STATUS_REFINE_FALLBACK 16
+
+int dpnegf_abi_version(void);
+const char* dpnegf_last_error(void);
+/* complex elements of one system's arena */
+int64_t dpnegf_arena_elems(int K, const int32_t* h_bsz);
+
+/* Transmission T(E,k) for P points, one workgroup per point.
+ * Replaces recursive_gf(...)[0] (dpnegf/negf/recursive_green_cal.py:324-477,
+ * forward sweep :198-207, g_trans :228) followed by DeviceProperty._cal_tc_
+ * (dpnegf/negf/device_property.py:374-406).
+ *   z[p]       = E + E_ref + i*eta_device                 (complex)
+ *   sigL[p]    = left self-energy  [nL x nL], nL <= n_0, added to hd_0's top-left corner
+ *   sigR[p]    = right self-energy [nR x nR], nR <= n_{K-1}, bottom-right of hd_{K-1}
+ *   sys_idx[p] = system (k-point) of point p; arena + sys * sys_stride (null: all 0)
+ *   ortho      = 1 if every sd = I and su = sl = 0 (S blocks are then not read)
+ *   precision  = DPNEGF_FP64 or DPNEGF_MIXED
+ *   T[p]       = Re Tr[Gamma_L G_{0,K-1} Gamma_R G_{0,K-1}^dagger]
+ *   status[p]  = DPNEGF_STATUS_* bits (may be null)
+ * Block sizes up to DPNEGF_TRANSMISSION_NMAX. */
+int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h_bsz,
+                            const int32_t* bsz, const void* arena, int64_t sys_stride, int nsys,
+                            const int32_t* sys_idx, int P, const void* z, const void* sigL, int nL,
+                            const void* sigR, int nR, int ortho, double* T, int32_t* status);
+
+/* Lopez-Sancho surface Green's function and lead self-energy for P points.
+ * Replaces selfEnergy (dpnegf/negf/surface_green.py:9-91) and its decimation
+ * cores (surface_green.py:102-159, 169-226), i.e. LeadProperty.self_energy_cal
+ * (dpnegf/negf/lead_property.py:266-384) over a batch of (E, k).
+ *   lead (per system, at lead + sys * lead_stride): hL, hLL, sL, sLL [n x n],
+ *        hDL, sDL [nred x n]   (the tuple of get_hs_lead,
+ *        dpnegf/negf/negf_hamiltonian_init.py:974, with HDL_reduced applied)
+ *   e[p]   = E + E_ref (complex allowed); the decimation uses e + i*eta
+ *   mode 0: Sigma = (e sDL - hDL) g_s (e sDL^H - hDL^H)      [nred x nred]
+ *   mode 1: Sigma = (e sL - hL) - g_s^{-1};  mode 2 (Bulk): g_s^{-1}
+ *   stop   = decimation threshold on max|alpha| + |beta| (reference: 1e-40)
+ *   eps_ws = workspace [P x n x n] complex; gs out [P x n x n]; sigma out [P x nred x nred]
+ *   iters/status/resid per point (may be null); status bits LS_* replace the
+ *   reference's RuntimeError / ArithmeticError / warning.
+ * n, nred up to DPNEGF_LEAD_NMAX. */
+int dpnegf_lead_self_energy(void* stream, int n, int nred, int mode, const void* lead,
+                            int64_t lead_stride, int nsys, const int32_t* sys_idx, int P,
+                            const void* e, double eta, double stop, void* eps_ws, void* gs,
+                            void* sigma, int ortho, int32_t* iters, int32_t* status,
+                            double* resid);
+
+/* Complex elements of the work buffer dpnegf_rgf_full needs. */
+int64_t dpnegf_rgf_full_workspace_elems(int K, const int32_t* h_bsz, int B);
+
+/* Every retarded block (and optionally lesser / greater) for B energies of one
+ * system.  Replaces recursive_gf_cal (dpnegf/negf/recursive_green_cal.py:4-296)
+ * including its lesser/greater quirks.  Per-block outputs are packed arenas,
+ * block q at the sum of the sizes of blocks < q:
+ *   gl, grd, gin, gnd, gip, gpd : B x n_q x n_q
+ *   grl, gnl, gpl               : B x n_{q+1} x n_q
+ *   gru, gnu, gpu               : B x n_q x n_{q+1}
+ *   lc (gr_lc, may be null)     : B x n_q x n_{K-1}
+ *   g_trans                     : B x n_0 x n_{K-1}
+ *   s_in / s_out (may be null: no lesser / greater): B x n_q x n_q arenas
+ * Block sizes up to DPNEGF_FULL_NMAX; B <= 65535 per call. */
+int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena, int B,
+                    const void* z, const void* sigL, int nL, const void* sigR, int nR, int ortho,
+                    void* gl, void* grd, void* grl, void* gru, void* lc, void* g_trans,
+                    const void* s_in, void* gin, void* gnd, void* gnl, void* gnu,
+                    const void* s_out, void* gip, void* gpd, void* gpl, void* gpu, void* work,
+                    int32_t* status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

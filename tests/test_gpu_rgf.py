@@ -1,0 +1,185 @@
+"""GPU parity of the RGF kernels against the CPU oracle (complex128).
+
+Bar: relative 1e-10 on every output block / T(E) (FP64 path).  The oracle is
+the reference restatement in ``oracle/`` (pinned by tests/test_oracle_*.py).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import device as odev, lead as olead, rgf as orgf, surface_green as osg
+from dpnegf_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch.device("cuda")
+
+
+def _rand_se(B, n, seed):
+    g = torch.Generator().manual_seed(seed)
+    a = torch.randn(B, n, n, generator=g, dtype=torch.float64) + 1j * torch.randn(
+        B, n, n, generator=g, dtype=torch.float64)
+    return 0.3 * a - 0.5j * torch.eye(n, dtype=torch.complex128)
+
+
+def _cmp(a, b, scale, name):
+    a = a.detach().cpu()
+    b = b.detach().cpu()
+    err = (a - b).abs().max().item() / scale
+    assert err < RTOL, f"{name}: rel err {err:.3e}"
+
+
+def _cmp_tuple(gpu, ref):
+    names = ["g_trans", "gr_lc", "grd", "grl", "gru", "gr_left", "gnd", "gnl", "gnu", "gin_left",
+             "gpd", "gpl", "gpu", "gip_left"]
+    for name, g, r in zip(names, gpu, ref):
+        if r is None:
+            assert g is None, name
+            continue
+        if torch.is_tensor(r):
+            _cmp(g, r, r.abs().max().item() + 1e-300, name)
+        else:
+            assert isinstance(g, list) and len(g) == len(r), name
+            scale = max([t.abs().max().item() for t in r] + [1e-300])
+            for q, (gq, rq) in enumerate(zip(g, r)):
+                assert gq.shape == rq.shape, (name, q)
+                _cmp(gq, rq, scale, f"{name}[{q}]")
+
+
+@pytest.mark.parametrize("sizes,overlap", [([8, 6, 8], False), ([5, 9, 3, 7], True), ([12], False),
+                                           ([16, 16, 16, 16, 16], False), ([40, 64, 33], True)])
+def test_recursive_gf_retarded_matches_oracle(sizes, overlap):
+    from dpnegf_amd.negf.recursive_green_cal import recursive_gf
+    dev = _dev()
+    print(f"case sizes={sizes} overlap={overlap}", flush=True)
+    hd, sd, hl, hu, sl, su = wl.random_btd(sizes, seed=1, overlap=overlap)
+    B = 5
+    sL, sR = _rand_se(B, sizes[0], 2), _rand_se(B, sizes[-1], 3)
+    E = torch.linspace(-1, 1, B, dtype=torch.float64) + 1e-3j
+    ref = orgf.recursive_gf(E, hl, hd, hu, sd, su, sl, sL, sR, eta=1e-4, need_gr_lc=True)
+    to = lambda xs: [x.to(dev) for x in xs]
+    out = recursive_gf(E.to(dev), to(hl), to(hd), to(hu), to(sd), to(su), to(sl), sL.to(dev),
+                       sR.to(dev), eta=1e-4, need_gr_lc=True)
+    _cmp_tuple(out, ref)
+
+
+def test_recursive_gf_lesser_greater_and_scalar_squeeze():
+    from dpnegf_amd.negf.recursive_green_cal import recursive_gf
+    dev = _dev()
+    sizes = [6, 4, 6]
+    hd, sd, hl, hu, sl, su = wl.random_btd(sizes, seed=4)
+    B = 3
+    sL, sR = _rand_se(B, 6, 5), _rand_se(B, 6, 6)
+    E = torch.tensor([-0.4, 0.1, 0.9], dtype=torch.complex128)
+    s_in = odev.build_s_in([(n, n) for n in sizes], olead.sigma2gamma(sL) * 0.7,
+                           olead.sigma2gamma(sR) * 0.2)
+    s_out = odev.build_s_in([(n, n) for n in sizes], olead.sigma2gamma(sL) * 0.3,
+                            olead.sigma2gamma(sR) * 0.8)
+    ref = orgf.recursive_gf(E, hl, hd, hu, sd, su, sl, sL, sR, eta=1e-5, s_in=s_in, s_out=s_out,
+                            need_lesser=True, need_greater=True, need_gr_lc=True)
+    to = lambda xs: [x.to(dev) for x in xs]
+    out = recursive_gf(E.to(dev), to(hl), to(hd), to(hu), to(sd), to(su), to(sl), sL.to(dev),
+                       sR.to(dev), eta=1e-5, s_in=to(s_in), s_out=to(s_out), need_lesser=True,
+                       need_greater=True, need_gr_lc=True)
+    _cmp_tuple(out, ref)
+    # scalar energy + 2-D self-energies squeeze the batch dim
+    ref1 = orgf.recursive_gf(E[1], hl, hd, hu, sd, su, sl, sL[1], sR[1], eta=1e-5)
+    out1 = recursive_gf(E[1].to(dev), to(hl), to(hd), to(hu), to(sd), to(su), to(sl),
+                        sL[1].to(dev), sR[1].to(dev), eta=1e-5)
+    assert out1[0].ndim == 2
+    _cmp_tuple(out1, ref1)
+
+
+def test_single_block_outputs_are_empty_lists():
+    from dpnegf_amd.negf.recursive_green_cal import recursive_gf
+    dev = _dev()
+    hd, sd, hl, hu, sl, su = wl.random_btd([7], seed=2)
+    sL = _rand_se(1, 7, 1)[0]
+    out = recursive_gf(torch.tensor(0.3, dtype=torch.complex128), [], [hd[0].to(dev)], [],
+                       [sd[0].to(dev)], [], [], sL.to(dev), sL.to(dev), eta=1e-5)
+    ref = orgf.recursive_gf(torch.tensor(0.3, dtype=torch.complex128), [], hd, [], sd, [], [], sL,
+                            sL, eta=1e-5)
+    assert out[3] == [] and out[4] == []
+    _cmp_tuple(out, ref)
+
+
+def test_cpu_tensors_are_rejected():
+    from dpnegf_amd.negf.recursive_green_cal import recursive_gf
+    hd, sd, hl, hu, sl, su = wl.random_btd([3, 3], seed=2)
+    with pytest.raises(RuntimeError, match="GPU only"):
+        recursive_gf(torch.tensor(0.0), hl, hd, hu, sd, su, sl, hd[0], hd[1])
+
+
+def _leads_sigmas_oracle(s, E, eta_lead):
+    out = []
+    for lead in (s.lead_L, s.lead_R):
+        hL, hLL, hDL, sL, sLL, sDL = lead
+        sig, gs, it = [], [], []
+        for e in E:
+            g_, n_, r_ = osg.lopez_sancho(hL.numpy(), hLL.numpy(), sL.numpy(), sLL.numpy(),
+                                          complex(e) + 1j * eta_lead)
+            it.append(n_)
+            gs.append(torch.tensor(g_))
+            sig.append(osg.selfEnergy(hL, hLL, sL, sLL, torch.tensor(e), hDL=hDL, sDL=sDL,
+                                      etaLead=eta_lead)[0])
+        out.append((torch.stack(sig), torch.stack(gs), it))
+    return out
+
+
+@pytest.mark.parametrize("name", ["chain2", "ribbon16", "ribbon48"])
+def test_lead_self_energy_matches_oracle(name):
+    from dpnegf_amd.negf.surface_green import self_energy_batched
+    dev = _dev()
+    if name == "chain2":
+        s, eta, E = wl.two_orbital_chain(4), 1e-5, np.linspace(-3, 3, 13)
+    elif name == "ribbon16":
+        s, eta, E = wl.ribbon(W=16, K=2, disorder=0.0), 1e-6, np.linspace(-3, 3, 9)
+    else:
+        s, eta, E = wl.ribbon(W=48, K=2, disorder=0.0), 1e-6, np.linspace(-3, 3, 5)
+    print(f"case {name}", flush=True)
+    ref = _leads_sigmas_oracle(s, E, eta)
+    for lead, (rsig, rgs, rit) in zip((s.lead_L, s.lead_R), ref):
+        res = self_energy_batched([tuple(x.to(dev) for x in lead)], torch.tensor(E), eta)
+        _cmp(res.gs, rgs, rgs.abs().max().item(), "g_s")
+        _cmp(res.sigma, rsig, rsig.abs().max().item(), "sigma")
+        its = res.iters.cpu().tolist()
+        assert max(abs(a - b) for a, b in zip(its, rit)) <= 1, (its, rit)
+
+
+def _T_oracle(s, E, eta_lead, eta_dev):
+    out = []
+    for e in E:
+        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e), hDL=l[2], sDL=l[5],
+                              etaLead=eta_lead)[0] for l in (s.lead_L, s.lead_R)]
+        ans = orgf.recursive_gf(torch.tensor(e, dtype=torch.complex128), s.hl, s.hd, s.hu, s.sd,
+                                s.su, s.sl, sig[0], sig[1], eta=eta_dev)
+        out.append(float(odev.transmission(ans[0], olead.sigma2gamma(sig[0]),
+                                           olead.sigma2gamma(sig[1]))))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("name", ["chain2", "ribbon16", "ribbon48", "uneven"])
+def test_transmission_pipeline_matches_oracle(name):
+    from dpnegf_amd.negf.device_property import transmission_pipeline
+    dev = _dev()
+    if name == "chain2":
+        s, eta, E = wl.two_orbital_chain(16), 1e-5, np.linspace(-3, 3, 21)
+    elif name == "ribbon16":
+        s, eta, E = wl.ribbon(W=16, K=12, seed=3), 1e-6, np.linspace(-3, 3, 11)
+    elif name == "ribbon48":
+        s, eta, E = wl.ribbon(W=48, K=6, seed=3), 1e-6, np.linspace(-3, 3, 5)
+    else:
+        s, eta, E = wl.multiorbital(widths=(4, 6, 5), K=5, norb=3, seed=2), 1e-5, np.linspace(-2, 2, 7)
+    print(f"case {name}", flush=True)
+    ref = _T_oracle(s, E, eta, eta)
+    T = transmission_pipeline(s.to(dev), torch.tensor(E), eta_lead=eta, eta_device=eta).cpu().numpy()
+    err = np.abs(T - ref).max() / max(1.0, np.abs(ref).max())
+    assert err < RTOL, (err, T, ref)
