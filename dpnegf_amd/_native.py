@@ -41,6 +41,13 @@ _SIGS = {
     "dpnegf_rgf_full_workspace_elems": ([_i32, _vp, _i32], _i64),
     "dpnegf_rgf_full": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _i32] + [_vp] * 17,
                         _i32),
+    "dpnegf_rgf_grad_workspace_elems": ([_i32, _vp, _i32], _i64),
+    "dpnegf_rgf_transmission_grad": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _i32,
+                                      _vp, _vp, _vp, _vp, _vp, _vp], _i32),
+    "dpnegf_weighted_re_sum": ([_vp, _vp, _vp, _vp, _i64, _i32, _i32], _i32),
+    "dpnegf_zgemm_batched": ([_vp, _i32, _i32, _i32, _i32, _vp, _i32, _i64, _i32, _vp, _i32, _i64,
+                              _i32, _f64, _f64, _f64, _f64, _vp, _i32, _i64], _i32),
+    "dpnegf_diag_product": ([_vp, _vp, _vp, _i64, _vp, _i64, _i32, _i32, _i32, _i32], _i32),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -3,19 +3,30 @@
 set -euo pipefail
 here="$(cd "$(dirname "$0")" && pwd)"
 out="$here/../_lib"
-mkdir -p "$out" "$here/../_build"
+bld="$here/../_build"
+mkdir -p "$out" "$bld"
 FLAGS="--offload-arch=gfx950 -O3 -std=c++17 -fPIC -Xclang -target-feature -Xclang -packed-fp32-ops -I$here -I$here/../../include"
 objs=()
+pids=()
 for f in rgf_transmission.hip lead_self_energy.hip rgf_full.hip capi.cpp; do
-  o="$here/../_build/${f%.*}.o"
+  o="$bld/${f%.*}.o"
+  rm -f "$o"
   if [[ "$f" == *.cpp ]]; then
     hipcc $FLAGS -x hip -c "$here/$f" -o "$o" &
   else
     hipcc $FLAGS -c "$here/$f" -o "$o" &
   fi
+  pids+=($!)
   objs+=("$o")
 done
-wait
+fail=0
+for p in "${pids[@]}"; do
+  wait "$p" || fail=1
+done
+if [[ $fail -ne 0 ]]; then
+  echo "build failed" >&2
+  exit 1
+fi
 hipcc --offload-arch=gfx950 -shared -fPIC "${objs[@]}" -o "$out/libdpnegf_amd.so.tmp"
 mv "$out/libdpnegf_amd.so.tmp" "$out/libdpnegf_amd.so"
 echo "built $out/libdpnegf_amd.so"

@@ -176,4 +176,74 @@ int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena
   return 0;
 }
 
+int64_t dpnegf_rgf_grad_workspace_elems(int K, const int32_t* h_bsz, int B) {
+  if (!h_bsz || K < 1 || B < 0) return -1;
+  return negf::rgf_grad_workspace(K, h_bsz, B);
+}
+
+int dpnegf_rgf_transmission_grad(void* stream, int K, const int32_t* h_bsz, const void* arena,
+                                 int B, const void* z, const void* sigL, int nL, const void* sigR,
+                                 int nR, int ortho, const double* w, void* grad_hd, void* grad_hu,
+                                 void* grad_hl, void* work, int32_t* status) {
+  int nmax = 0;
+  if (int r = check_blocks(K, h_bsz, DPNEGF_FULL_NMAX, &nmax)) return r;
+  if (B < 0) return fail(DPNEGF_ERR_ARG, "B must be >= 0");
+  if (B == 0) return 0;
+  if (B > 65535) return fail(DPNEGF_ERR_ARG, "B=%d exceeds 65535 per call", B);
+  if (!arena || !z || !sigL || !sigR || !w || !grad_hd || !work)
+    return fail(DPNEGF_ERR_ARG, "null device pointer");
+  if (K > 1 && (!grad_hu || !grad_hl)) return fail(DPNEGF_ERR_ARG, "grad_hu/grad_hl required");
+  if (nL < 1 || nL > h_bsz[0] || nR < 1 || nR > h_bsz[K - 1])
+    return fail(DPNEGF_ERR_ARG, "self-energy larger than its corner block");
+  negf::RgfGradArgs a;
+  a.K = K; a.B = B; a.ortho = ortho; a.bsz = h_bsz; a.arena = (const cx<double>*)arena;
+  a.z = (const cx<double>*)z; a.sigL = (const cx<double>*)sigL; a.nL = nL;
+  a.sigR = (const cx<double>*)sigR; a.nR = nR; a.w = w; a.grad_hd = (cx<double>*)grad_hd;
+  a.grad_hu = (cx<double>*)grad_hu; a.grad_hl = (cx<double>*)grad_hl; a.work = (cx<double>*)work;
+  a.status = status;
+  hipError_t e = negf::rgf_grad(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "rgf_transmission_grad");
+  return 0;
+}
+
+int dpnegf_weighted_re_sum(void* stream, double* out, const void* in, const void* c,
+                           int64_t count, int B, int accumulate) {
+  if (count < 0 || B < 0) return fail(DPNEGF_ERR_ARG, "negative size");
+  if (count == 0) return 0;
+  if (!out || (B > 0 && (!in || !c))) return fail(DPNEGF_ERR_ARG, "null device pointer");
+  hipError_t e = negf::weighted_re_sum(out, (const cx<double>*)in, (const cx<double>*)c, count, B,
+                                       accumulate, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "weighted_re_sum");
+  return 0;
+}
+
+int dpnegf_zgemm_batched(void* stream, int B, int m, int n, int k, const void* A, int lda,
+                         int64_t strideA, int opA, const void* Bm, int ldb, int64_t strideB, int opB,
+                         double alpha_re, double alpha_im, double beta_re, double beta_im, void* C,
+                         int ldc, int64_t strideC) {
+  if (B < 0 || m < 0 || n < 0 || k < 0) return fail(DPNEGF_ERR_ARG, "negative size");
+  if (B > 65535) return fail(DPNEGF_ERR_ARG, "B=%d exceeds 65535 per call", B);
+  if (B == 0 || m == 0 || n == 0) return 0;
+  if (!A || !Bm || !C) return fail(DPNEGF_ERR_ARG, "null device pointer");
+  if (opA < 0 || opA > 3 || opB < 0 || opB > 3) return fail(DPNEGF_ERR_ARG, "op must be 0..3");
+  hipError_t e = negf::zgemm_batched(B, m, n, k, (const cx<double>*)A, lda, strideA, opA,
+                                     (const cx<double>*)Bm, ldb, strideB, opB,
+                                     cx<double>{alpha_re, alpha_im}, cx<double>{beta_re, beta_im},
+                                     (cx<double>*)C, ldc, strideC, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "zgemm_batched");
+  return 0;
+}
+
+int dpnegf_diag_product(void* stream, void* out, const void* A, int64_t strideA, const void* Bm,
+                        int64_t strideB, int m, int n, int B, int accumulate) {
+  if (B < 0 || m < 0 || n < 0) return fail(DPNEGF_ERR_ARG, "negative size");
+  if (B == 0 || m == 0) return 0;
+  if (!out || !A || !Bm) return fail(DPNEGF_ERR_ARG, "null device pointer");
+  hipError_t e = negf::diag_product((cx<double>*)out, (const cx<double>*)A, strideA,
+                                    (const cx<double>*)Bm, strideB, m, n, B, accumulate,
+                                    (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "diag_product");
+  return 0;
+}
+
 }  // extern "C"

@@ -13,6 +13,7 @@
 // form H - z_b S), binv (Gauss-Jordan inverse, one workgroup per matrix),
 // bbuild (-(H - z_b S) - Sigma corners).
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "rgf_kernels.hpp"
@@ -98,6 +99,100 @@ __global__ void bconjt_kernel(cx<double>* D, const cx<double>* S, int m, int n, 
   }
 }
 
+
+// out[i][k] = sum_b conj(in[b][k][i]);  in: [B, m, n], out: [n, m]
+__global__ void breduce_conjt_kernel(cx<double>* out, const cx<double>* in, int m, int n, int B) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m * n) return;
+  const int k = e / n, i = e % n;  // in element (k, i)
+  cx<double> acc{0, 0};
+  for (int b = 0; b < B; ++b) acc = acc + in[(int64_t)b * m * n + e];
+  out[(int64_t)i * m + k] = conj(acc);
+}
+
+// Gamma = i (Sigma - Sigma^H) padded into an n x n block (top-left or
+// bottom-right corner), scaled per batch by scale[b] (null: 1).
+__global__ void bgamma_pad_kernel(cx<double>* D, const cx<double>* sig, int ns, int n, int B,
+                                  int at_end, const double* scale, double factor) {
+  const int64_t tot = (int64_t)B * n * n;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < tot;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(x / ((int64_t)n * n));
+    const int e = (int)(x % ((int64_t)n * n));
+    int i = e / n, k = e % n;
+    cx<double> v{0, 0};
+    const int o = at_end ? n - ns : 0;
+    if (i >= o && k >= o && i < o + ns && k < o + ns) {
+      v = gamma_of(sig + (int64_t)b * ns * ns, ns, i - o, k - o);
+      const double f = factor * (scale ? scale[b] : 1.0);
+      v.re *= f;
+      v.im *= f;
+    }
+    D[x] = v;
+  }
+}
+
+// out[e] (+)= sum_b Re(c_b * in[b][e]) over a batch of B matrices of `count` elements
+__global__ void bweighted_re_sum_kernel(double* out, const cx<double>* in, const cx<double>* c,
+                                        int64_t count, int B, int accumulate) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int b = 0; b < B; ++b) {
+      const cx<double> v = in[(int64_t)b * count + e];
+      acc += c[b].re * v.re - c[b].im * v.im;
+    }
+    out[e] = accumulate ? out[e] + acc : acc;
+  }
+}
+
+static hipError_t gemm(hipStream_t st, int B, int m, int n, int k, Opnd A, Opnd Bo,
+                       cx<double> alpha, cx<double> beta, cx<double>* C, int64_t cstride,
+                       int ldc, const cx<double>* z);
+static Opnd mat(const cx<double>* h, int ld, int64_t bstride, int op, const cx<double>* s,
+                int64_t sstride);
+
+// out[b][i] (+)= sum_j A[b][i][j] * B[b][j][i]   (A: m x n, B: n x m; diagonal of A B)
+__global__ void bdiag_product_kernel(cx<double>* out, const cx<double>* A, int64_t sa,
+                                     const cx<double>* Bm, int64_t sb, int m, int n, int B,
+                                     int accumulate) {
+  const int64_t tot = (int64_t)B * m;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < tot;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = x / m;
+    const int i = (int)(x % m);
+    const cx<double>* a = A + b * sa + (int64_t)i * n;
+    const cx<double>* bb = Bm + b * sb + i;
+    cx<double> acc{0, 0};
+    for (int j = 0; j < n; ++j) acc = acc + a[j] * bb[(int64_t)j * m];
+    out[x] = accumulate ? out[x] + acc : acc;
+  }
+}
+
+hipError_t diag_product(cx<double>* out, const cx<double>* A, int64_t sa, const cx<double>* Bm,
+                        int64_t sb, int m, int n, int B, int accumulate, hipStream_t st) {
+  const int grid = (int)std::min<int64_t>(((int64_t)B * m + 255) / 256, 65535);
+  hipLaunchKernelGGL(bdiag_product_kernel, dim3(grid), dim3(256), 0, st, out, A, sa, Bm, sb, m, n,
+                     B, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t zgemm_batched(int B, int m, int n, int k, const cx<double>* A, int lda, int64_t sa,
+                         int opA, const cx<double>* Bm, int ldb, int64_t sb, int opB,
+                         cx<double> alpha, cx<double> beta, cx<double>* C, int ldc, int64_t sc,
+                         hipStream_t st) {
+  return gemm(st, B, m, n, k, mat(A, lda, sa, opA, nullptr, 0), mat(Bm, ldb, sb, opB, nullptr, 0), alpha,
+              beta, C, sc, ldc, nullptr);
+}
+
+hipError_t weighted_re_sum(double* out, const cx<double>* in, const cx<double>* c, int64_t count,
+                           int B, int accumulate, hipStream_t st) {
+  const int grid = (int)std::min<int64_t>((count + 255) / 256, 65535);
+  hipLaunchKernelGGL(bweighted_re_sum_kernel, dim3(grid), dim3(256), 0, st, out, in, c, count, B,
+                     accumulate);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------------
 // host side
 
@@ -126,7 +221,9 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
 }
 
 static Opnd mat(const cx<double>* h, int ld, int64_t bstride, int op = OP_N,
-                const cx<double>* s = nullptr, int64_t sstride = 0) {
+                const cx<double>* s = nullptr, int64_t sstride = 0);
+static Opnd mat(const cx<double>* h, int ld, int64_t bstride, int op, const cx<double>* s,
+                int64_t sstride) {
   Opnd o;
   o.h = h; o.s = s; o.bstride = bstride; o.sstride = sstride; o.ld = ld; o.op = op;
   return o;
@@ -303,6 +400,129 @@ hipError_t rgf_full(const RgfFullArgs& a, hipStream_t st) {
         CHK(hipGetLastError());
       }
     }
+  }
+  return hipSuccess;
+}
+
+
+int64_t rgf_grad_workspace(int K, const int32_t* bsz, int B) {
+  int64_t nsq = 0, nx = 0, nlc = 0;
+  int nmax = 0;
+  for (int q = 0; q < K; ++q) {
+    nsq += (int64_t)bsz[q] * bsz[q];
+    nlc += (int64_t)bsz[q] * bsz[K - 1];
+    if (q + 1 < K) nx += (int64_t)bsz[q] * bsz[q + 1];
+    nmax = std::max(nmax, (int)bsz[q]);
+  }
+  const int64_t m2 = (int64_t)nmax * nmax;
+  // gl, grd | grl, gru | lc | g_trans | work(5) | gammaL, gammaR, C | R x2, G0 x2, D x2, W
+  return B * (2 * nsq + 2 * std::max<int64_t>(nx, 1) + nlc + m2 + 5 * m2 + 3 * m2 + 7 * m2);
+}
+
+hipError_t rgf_grad(const RgfGradArgs& g, hipStream_t st) {
+  const int K = g.K, B = g.B;
+  const std::vector<int> n(g.bsz, g.bsz + K);
+  int64_t nsq = 0, nx = 0, nlc = 0;
+  int nmax = 0;
+  std::vector<int64_t> od(K), ou(K > 1 ? K - 1 : 1), olc(K), hoff(K), gsq(K), gx(K > 1 ? K - 1 : 1);
+  int64_t sh = 0, s_sq = 0, s_x = 0;
+  for (int q = 0; q < K; ++q) {
+    od[q] = (int64_t)B * nsq; gsq[q] = s_sq;
+    nsq += (int64_t)n[q] * n[q]; s_sq += (int64_t)n[q] * n[q];
+    olc[q] = (int64_t)B * nlc; nlc += (int64_t)n[q] * n[K - 1];
+    hoff[q] = sh; sh += 2 * (int64_t)n[q] * n[q];
+    if (q + 1 < K) {
+      ou[q] = (int64_t)B * nx; gx[q] = s_x;
+      nx += (int64_t)n[q] * n[q + 1]; s_x += (int64_t)n[q] * n[q + 1];
+      sh += 4 * (int64_t)n[q] * n[q + 1];
+    }
+    nmax = std::max(nmax, n[q]);
+  }
+  const int64_t m2 = (int64_t)B * nmax * nmax;
+  cx<double>* w = g.work;
+  RgfFullArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.K = K; a.B = B; a.nmax = nmax; a.ortho = g.ortho; a.bsz = g.bsz; a.arena = g.arena; a.z = g.z;
+  a.sigL = g.sigL; a.nL = g.nL; a.sigR = g.sigR; a.nR = g.nR; a.status = g.status;
+  a.gl = w; w += (int64_t)B * nsq;
+  a.grd = w; w += (int64_t)B * nsq;
+  a.grl = w; w += (int64_t)B * std::max<int64_t>(nx, 1);
+  a.gru = w; w += (int64_t)B * std::max<int64_t>(nx, 1);
+  a.lc = w; w += (int64_t)B * nlc;
+  a.g_trans = w; w += m2;
+  a.work = w; w += 5 * m2;
+  CHK(rgf_full(a, st));
+  cx<double>* gamL = w; w += m2;
+  cx<double>* gamR = w; w += m2;
+  cx<double>* C = w; w += m2;
+  cx<double>* Rb[2] = {w, w + m2}; w += 2 * m2;
+  cx<double>* G0[2] = {w, w + m2}; w += 2 * m2;
+  cx<double>* D[2] = {w, w + m2}; w += 2 * m2;
+  cx<double>* Wm = w; w += m2;
+  const cx<double> one{1, 0}, zero{0, 0};
+  const cx<double>* H = g.arena;
+  auto hu = [&](int q) { return H + hoff[q] + 2 * (int64_t)n[q] * n[q]; };
+  auto suu = [&](int q) { return hu(q) + (int64_t)n[q] * n[q + 1]; };
+  auto Uop = [&](int q) { return mat(hu(q), n[q + 1], 0, OP_N, g.ortho ? nullptr : suu(q), 0); };
+  const int n0 = n[0], nK = n[K - 1];
+  auto lc = [&](int q) { return a.lc + olc[q]; };
+  auto gl = [&](int q) { return a.gl + od[q]; };
+  auto grd = [&](int q) { return a.grd + od[q]; };
+  auto launch_grid = [](int64_t tot) { return (int)std::min<int64_t>((tot + 255) / 256, 65535); };
+
+  // C = Gamma_R G_{0,K-1}^H (2 w Gamma_L)      [nK x n0]
+  hipLaunchKernelGGL(bgamma_pad_kernel, dim3(launch_grid((int64_t)B * n0 * n0)), dim3(256), 0, st,
+                     gamL, g.sigL, g.nL, n0, B, 0, g.w, 2.0);
+  hipLaunchKernelGGL(bgamma_pad_kernel, dim3(launch_grid((int64_t)B * nK * nK)), dim3(256), 0, st,
+                     gamR, g.sigR, g.nR, nK, B, 1, (const double*)nullptr, 1.0);
+  CHK(hipGetLastError());
+  CHK(gemm(st, B, nK, n0, nK, mat(gamR, nK, (int64_t)nK * nK), mat(lc(0), nK, (int64_t)n0 * nK, OP_H),
+           one, zero, Wm, (int64_t)nK * n0, n0, g.z));
+  CHK(gemm(st, B, nK, n0, n0, mat(Wm, n0, (int64_t)nK * n0), mat(gamL, n0, (int64_t)n0 * n0), one,
+           zero, C, (int64_t)nK * n0, n0, g.z));
+  // first block row: G_{0,0} = grd_0; G_{0,b} = R_b G_bb, R_1 = g_0 U_0, R_{b+1} = R_b g_b U_b
+  auto reduce = [&](cx<double>* out, const cx<double>* in, int m_, int n_) {
+    hipLaunchKernelGGL(breduce_conjt_kernel, dim3((m_ * n_ + 255) / 256), dim3(256), 0, st, out,
+                       in, m_, n_, B);
+    return hipGetLastError();
+  };
+  const cx<double>* g0b = grd(0);  // G_{0,0}
+  int cur = 0;
+  // D_0 = C G_{0,0}
+  CHK(gemm(st, B, nK, n0, n0, mat(C, n0, (int64_t)nK * n0), mat(g0b, n0, (int64_t)n0 * n0), one,
+           zero, D[0], (int64_t)nK * n0, n0, g.z));
+  for (int q = 0; q < K; ++q) {
+    const int nq = n[q];
+    // grad_hd[q] = sum_b (G_{q,K-1} D_q)^H
+    CHK(gemm(st, B, nq, nq, nK, mat(lc(q), nK, (int64_t)nq * nK), mat(D[cur], nq, (int64_t)nK * nq),
+             one, zero, Wm, (int64_t)nq * nq, nq, g.z));
+    CHK(reduce(g.grad_hd + gsq[q], Wm, nq, nq));
+    if (q + 1 == K) break;
+    const int n1 = n[q + 1];
+    // R_{q+1} = (q == 0 ? g_0 : R_q g_q) U_q
+    const cx<double>* left = gl(0);
+    int64_t lstride = (int64_t)n0 * n0;
+    if (q > 0) {
+      CHK(gemm(st, B, n0, nq, nq, mat(Rb[cur], nq, (int64_t)n0 * nq), mat(gl(q), nq, (int64_t)nq * nq),
+               one, zero, G0[1 - cur], (int64_t)n0 * nq, nq, g.z));
+      left = G0[1 - cur];
+      lstride = (int64_t)n0 * nq;
+    }
+    CHK(gemm(st, B, n0, n1, nq, mat(left, nq, lstride), Uop(q), one, zero, Rb[1 - cur],
+             (int64_t)n0 * n1, n1, g.z));
+    // G_{0,q+1} = R_{q+1} G_{q+1,q+1};  D_{q+1} = C G_{0,q+1}
+    CHK(gemm(st, B, n0, n1, n1, mat(Rb[1 - cur], n1, (int64_t)n0 * n1),
+             mat(grd(q + 1), n1, (int64_t)n1 * n1), one, zero, G0[cur], (int64_t)n0 * n1, n1, g.z));
+    CHK(gemm(st, B, nK, n1, n0, mat(C, n0, (int64_t)nK * n0), mat(G0[cur], n1, (int64_t)n0 * n1), one,
+             zero, D[1 - cur], (int64_t)nK * n1, n1, g.z));
+    // grad_hu[q] = sum_b (G_{q+1,K-1} D_q)^H ;  grad_hl[q] = sum_b (G_{q,K-1} D_{q+1})^H
+    CHK(gemm(st, B, n1, nq, nK, mat(lc(q + 1), nK, (int64_t)n1 * nK), mat(D[cur], nq, (int64_t)nK * nq),
+             one, zero, Wm, (int64_t)n1 * nq, nq, g.z));
+    CHK(reduce(g.grad_hu + gx[q], Wm, n1, nq));
+    CHK(gemm(st, B, nq, n1, nK, mat(lc(q), nK, (int64_t)nq * nK), mat(D[1 - cur], n1, (int64_t)nK * n1),
+             one, zero, Wm, (int64_t)nq * n1, n1, g.z));
+    CHK(reduce(g.grad_hl + gx[q], Wm, nq, n1));
+    cur = 1 - cur;
   }
   return hipSuccess;
 }

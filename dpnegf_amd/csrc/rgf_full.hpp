@@ -50,4 +50,30 @@ struct RgfFullArgs {
 
 hipError_t rgf_full(const RgfFullArgs& a, hipStream_t st);
 
+struct RgfGradArgs {
+  int K, B, ortho;
+  const int32_t* bsz;          // host [K]
+  const cx<double>* arena;
+  const cx<double>* z;
+  const cx<double>* sigL; int nL;
+  const cx<double>* sigR; int nR;
+  const double* w;             // [B] weights of T(E_b) in the objective
+  cx<double>* grad_hd;         // [sum_q n_q^2]
+  cx<double>* grad_hu;         // [sum_q n_q n_q+1]
+  cx<double>* grad_hl;         // [sum_q n_q+1 n_q]
+  cx<double>* work;            // rgf_grad_workspace elements
+  int32_t* status;
+};
+
+hipError_t diag_product(cx<double>* out, const cx<double>* A, int64_t sa, const cx<double>* Bm,
+                        int64_t sb, int m, int n, int B, int accumulate, hipStream_t st);
+hipError_t zgemm_batched(int B, int m, int n, int k, const cx<double>* A, int lda, int64_t sa,
+                         int opA, const cx<double>* Bm, int ldb, int64_t sb, int opB,
+                         cx<double> alpha, cx<double> beta, cx<double>* C, int ldc, int64_t sc,
+                         hipStream_t st);
+hipError_t weighted_re_sum(double* out, const cx<double>* in, const cx<double>* c, int64_t count,
+                           int B, int accumulate, hipStream_t st);
+int64_t rgf_grad_workspace(int K, const int32_t* bsz, int B);
+hipError_t rgf_grad(const RgfGradArgs& a, hipStream_t st);
+
 }  // namespace negf

@@ -1,0 +1,77 @@
+"""Differentiable transmission: fused forward kernel + hand-written adjoint.
+
+``transmission(...)`` returns T(E_b) for B energies of one system and is a
+``torch.autograd.Function``: ``backward`` receives dJ/dT_b and calls
+``dpnegf_rgf_transmission_grad``, which returns dJ/dH for every on-site block
+``hd_q`` and coupling block ``hu_q`` / ``hl_q`` summed over energies (torch's
+complex-gradient convention, dJ/dRe + i dJ/dIm).  The reference has no such
+kernel; it relies on autograd through ``recursive_gf``
+(``dpnegf/negf/recursive_green_cal.py:89-112``), which is what the tests
+compare against, together with central finite differences.
+"""
+import torch
+
+from .. import _native as nat
+from ._btd import C128, block_sizes, is_orthogonal, pack_arena, split_blocks
+from .recursive_green_cal import BTDSystems, rgf_transmission
+
+
+class _Transmission(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, meta, z, sigL, sigR, *blocks):
+        K, sd, su, sl, precision = meta
+        hd = list(blocks[:K])
+        hu = list(blocks[K:2 * K - 1])
+        hl = list(blocks[2 * K - 1:3 * K - 2])
+        dev = z.device
+        bs = BTDSystems([(hd, hu, hl, sd, su, sl)], dev)
+        T = rgf_transmission(bs, z, sigL, sigR, precision=precision)
+        ctx.K = K
+        ctx.bs = bs
+        ctx.save_for_backward(z, sigL, sigR)
+        return T
+
+    @staticmethod
+    def backward(ctx, gT):
+        z, sigL, sigR = ctx.saved_tensors
+        bs = ctx.bs
+        K = ctx.K
+        sizes = bs.sizes
+        dev = z.device
+        B = z.shape[0]
+        w = gT.detach().to(device=dev, dtype=torch.float64).contiguous()
+        h_bsz = nat.host_i32(sizes)
+        nwork = nat.load().dpnegf_rgf_grad_workspace_elems(K, h_bsz, B)
+        work = torch.empty(nwork, dtype=C128, device=dev)
+        nsq = sum(n * n for n in sizes)
+        nx = sum(sizes[q] * sizes[q + 1] for q in range(K - 1))
+        ghd = torch.empty(nsq, dtype=C128, device=dev)
+        ghu = torch.empty(max(nx, 1), dtype=C128, device=dev)
+        ghl = torch.empty(max(nx, 1), dtype=C128, device=dev)
+        status = torch.zeros(B, dtype=torch.int32, device=dev)
+        nat.call("dpnegf_rgf_transmission_grad", nat.stream_ptr(dev), K, h_bsz, nat.ptr(bs.arena), B,
+                 nat.ptr(z), nat.ptr(sigL), sigL.shape[-1], nat.ptr(sigR), sigR.shape[-1],
+                 int(bs.ortho), nat.ptr(w), nat.ptr(ghd), nat.ptr(ghu), nat.ptr(ghl),
+                 nat.ptr(work), nat.ptr(status))
+        g_hd = [t[0] for t in split_blocks(ghd, 1, [(n, n) for n in sizes])]
+        g_hu = [t[0] for t in split_blocks(ghu, 1, [(sizes[q], sizes[q + 1]) for q in range(K - 1)])]
+        g_hl = [t[0] for t in split_blocks(ghl, 1, [(sizes[q + 1], sizes[q]) for q in range(K - 1)])]
+        return (None, None, None, None, *g_hd, *g_hu, *g_hl)
+
+
+def transmission(energy, hd, hu, hl, sd, su, sl, sigL, sigR, eta, E_ref=0.0, precision="fp64"):
+    """Differentiable T(E) [B] of one system.
+
+    ``energy`` [B] relative to ``E_ref``; ``sigL`` [B, nL, nL] / ``sigR``
+    [B, nR, nR] the lead self-energies trimmed to their corners (treated as
+    constants).  Gradients flow to ``hd``, ``hu``, ``hl``.
+    """
+    dev = hd[0].device
+    nat.require_device(hd[0])
+    K = len(hd)
+    z = (torch.as_tensor(energy).to(device=dev, dtype=C128).reshape(-1) + E_ref + 1j * eta)
+    z = z.contiguous()
+    meta = (K, [s.detach() for s in sd], [s.detach() for s in su], [s.detach() for s in sl],
+            precision)
+    return _Transmission.apply(meta, z, sigL.detach().contiguous(), sigR.detach().contiguous(),
+                               *hd, *hu, *hl)

@@ -1,0 +1,104 @@
+"""Lead self-energies on the GPU (``dpnegf.negf.lead_property`` interface).
+
+:class:`LeadProperty` keeps the reference's constructor, attributes and
+methods (``dpnegf/negf/lead_property.py:30-460``): ``self_energy`` /
+``self_energy_cal`` compute Sigma with the batched Lopez-Sancho kernel (a 1-D
+energy tensor gives [B, n, n] in one launch instead of the joblib pool of
+``compute_all_self_energy``, ``:599-683``), ``HDL_reduced``,
+``sigmaLR2Gamma``, ``fermi_dirac`` and ``gamma`` are unchanged.  The HDF5
+self-energy cache (``:932-984``) and Bloch unfolding (``:330-379``) are not
+part of this package.
+"""
+import numpy as np
+import torch
+
+from ._btd import C128
+from .surface_green import self_energy_batched
+
+_EV2J = 1.6021766208e-19
+_BOLTZMANN = 1.380649e-23
+
+
+class LeadProperty(object):
+    def __init__(self, tab, hamiltonian, structure, results_path, voltage, structure_leads_fold=None,
+                 bloch_sorted_indice=None, useBloch=False, bloch_factor=[1, 1, 1], bloch_R_list=None,
+                 e_T=300, efermi=0.0, E_ref=None, device="cuda"):
+        self.hamiltonian = hamiltonian
+        self.structure = structure
+        self.tab = tab
+        self.voltage = voltage
+        self.results_path = results_path
+        self.kBT = _BOLTZMANN * e_T / _EV2J
+        self.e_T = e_T
+        self.efermi = efermi
+        self.E_ref = efermi if E_ref is None else E_ref
+        self.chemiPot_lead = efermi - voltage
+        self.kpoint = None
+        self.voltage_old = None
+        self.useBloch = useBloch
+        if useBloch:
+            raise NotImplementedError("Bloch-unfolded leads are not supported by dpnegf_amd")
+        self.device = torch.device(device)
+
+    def _lead_blocks(self, kpoint):
+        kp = torch.tensor(kpoint)
+        if (not hasattr(self, "HLk")) or self.voltage_old is None \
+                or abs(self.voltage_old - self.voltage) > 1e-6 \
+                or max(abs(self.kpoint - kp)) > 1e-6:
+            blocks = self.hamiltonian.get_hs_lead(kpoint, tab=self.tab, v=self.voltage)
+            self.HLk, self.HLLk, self.HDLk, self.SLk, self.SLLk, self.SDLk = [
+                b.to(device=self.device, dtype=C128) for b in blocks]
+            self.voltage_old = self.voltage
+            self.kpoint = kp
+        subblocks = None
+        if hasattr(self.hamiltonian, "get_hs_device"):
+            try:
+                subblocks = self.hamiltonian.get_hs_device(kpoint, only_subblocks=True)
+            except TypeError:
+                subblocks = None
+        hdl, sdl = self.HDL_reduced(self.HDLk, self.SDLk, subblocks)
+        return (self.HLk, self.HLLk, hdl, self.SLk, self.SLLk, sdl)
+
+    def self_energy(self, kpoint, energy, eta_lead=1e-5, method="Lopez-Sancho", save_path=None,
+                    save_format="h5", se_info_display=False, HS_inmem=True):
+        """Sets ``self.se``: [a, a] for a scalar energy, [B, a, a] for a 1-D one."""
+        assert len(np.array(kpoint).reshape(-1)) == 3
+        self.se = self.self_energy_cal(kpoint, energy, eta_lead=eta_lead, method=method,
+                                       HS_inmem=HS_inmem)
+
+    def self_energy_cal(self, kpoint, energy, eta_lead=1e-5, method="Lopez-Sancho",
+                        se_numba_jit=None, HS_inmem=True):
+        lead = self._lead_blocks(kpoint)
+        e = torch.as_tensor(energy)
+        scalar = e.ndim == 0 or (e.ndim == 1 and e.numel() == 1 and torch.as_tensor(energy).ndim == 0)
+        res = self_energy_batched([lead], e.reshape(-1), eta_lead, E_ref=self.E_ref)
+        self.se = res.sigma[0] if e.ndim == 0 else res.sigma
+        if not HS_inmem:
+            del self.HLk, self.HLLk, self.HDLk, self.SLk, self.SLLk, self.SDLk
+        return self.se
+
+    @staticmethod
+    def HDL_reduced(HDL, SDL, subblocks):
+        """``lead_property.py:386-434``."""
+        assert len(HDL.shape) == 2 and len(SDL.shape) == 2 and HDL.shape == SDL.shape
+        nz = HDL.nonzero()
+        lo, hi = nz.min(dim=0).values, nz.max(dim=0).values
+        cut = (lo, hi) if subblocks is None else ((subblocks[-1], subblocks[-1]),
+                                                   (subblocks[0], subblocks[0]))
+        if HDL.shape[0] == 1:
+            return HDL, SDL
+        if lo[0] > 0:
+            r = int(cut[0][0]) if subblocks is None else -1 * int(cut[0][0])
+            return HDL[r:, :], SDL[r:, :]
+        r = int(cut[1][0]) + 1 if subblocks is None else int(cut[1][0])
+        return HDL[:r, :], SDL[:r, :]
+
+    def sigmaLR2Gamma(self, se):
+        return 1j * (se - se.mH)
+
+    def fermi_dirac(self, x):
+        return 1 / (1 + torch.exp((x - self.chemiPot_lead) / self.kBT))
+
+    @property
+    def gamma(self):
+        return self.sigmaLR2Gamma(self.se)

@@ -83,47 +83,15 @@ def recursive_gf(energy, hl, hd, hu, sd, su, sl, left_se, right_se, seP=None, E_
     if isinstance(right_se, torch.Tensor):
         sigR, nR = corner_self_energy(right_se, sizes[-1], B, True, device)
 
+    raw = _run_full(z, arena, sizes, ortho, sigL, nL, sigR, nR, need_gr_lc,
+                    pack_batched_blocks(s_in, B, device) if need_lesser else None,
+                    pack_batched_blocks(s_out, B, device) if need_greater else None)
     sq_shapes = [(n, n) for n in sizes]
     lo_shapes = [(sizes[q + 1], sizes[q]) for q in range(K - 1)]
     up_shapes = [(sizes[q], sizes[q + 1]) for q in range(K - 1)]
     lc_shapes = [(n, sizes[-1]) for n in sizes]
-    nsq = sum(n * n for n in sizes)
-    nx = sum(a * b for a, b in up_shapes)
-    nlc = sum(n * sizes[-1] for n in sizes)
-
-    def alloc(count):
-        return torch.empty(B * count, dtype=C128, device=device)
-
-    gl, grd = alloc(nsq), alloc(nsq)
-    grl, gru = alloc(max(nx, 1)), alloc(max(nx, 1))
-    lc = alloc(nlc) if need_gr_lc else None
-    g_trans = torch.empty(B, sizes[0], sizes[-1], dtype=C128, device=device)
-    lesser = greater = None
-    if need_lesser:
-        sin = pack_batched_blocks(s_in, B, device)
-        lesser = (sin, alloc(nsq), alloc(nsq), alloc(max(nx, 1)), alloc(max(nx, 1)))
-    if need_greater:
-        sout = pack_batched_blocks(s_out, B, device)
-        greater = (sout, alloc(nsq), alloc(nsq), alloc(max(nx, 1)), alloc(max(nx, 1)))
-    status = torch.zeros(B, dtype=torch.int32, device=device)
-
-    if B > _MAX_B:
-        raise ValueError(f"at most {_MAX_B} energies per recursive_gf call (got {B}); "
-                         "split the energy grid into chunks as the runner does")
-    nmax = max(sizes)
-    work = torch.empty(5 * B * nmax * nmax, dtype=C128, device=device)
-    args = [nat.stream_ptr(device), K, nat.host_i32(sizes), nat.ptr(arena), B, nat.ptr(z),
-            nat.ptr(sigL), nL, nat.ptr(sigR), nR, int(ortho),
-            nat.ptr(gl), nat.ptr(grd), nat.ptr(grl), nat.ptr(gru), nat.ptr(lc),
-            nat.ptr(g_trans)]
-    for part in (lesser, greater):
-        args += [nat.ptr(t) for t in part] if part else [nat.ptr(None)] * 5
-    args += [nat.ptr(work), nat.ptr(status)]
-    nat.call("dpnegf_rgf_full", *args)
-
-    if int(status.max().item()) & nat.STATUS_SINGULAR:
-        raise torch.linalg.LinAlgError("singular block in the recursive Green's function sweep")
-
+    gl, grd, grl, gru, lc, g_trans = (raw[k] for k in ("gl", "grd", "grl", "gru", "lc", "g_trans"))
+    lesser, greater = raw["lesser"], raw["greater"]
     grd_l = split_blocks(grd, B, sq_shapes)
     grl_l = split_blocks(grl, B, lo_shapes)
     gru_l = split_blocks(gru, B, up_shapes)
@@ -143,6 +111,43 @@ def recursive_gf(energy, hl, hd, hu, sd, su, sl, left_se, right_se, seP=None, E_
     if squeezed:
         ans = _squeeze_ans(ans)
     return ans
+
+
+def _run_full(z, arena, sizes, ortho, sigL, nL, sigR, nR, need_lc, s_in=None, s_out=None):
+    """One ``dpnegf_rgf_full`` call; returns the raw block-major arenas."""
+    device = z.device
+    B = z.shape[0]
+    K = len(sizes)
+    if B > _MAX_B:
+        raise ValueError(f"at most {_MAX_B} energies per call (got {B}); "
+                         "split the energy grid into chunks as the runner does")
+    nsq = sum(n * n for n in sizes)
+    nx = sum(sizes[q] * sizes[q + 1] for q in range(K - 1))
+    nlc = sum(n * sizes[-1] for n in sizes)
+
+    def alloc(count):
+        return torch.empty(B * max(count, 1), dtype=C128, device=device)
+
+    out = {"gl": alloc(nsq), "grd": alloc(nsq), "grl": alloc(nx), "gru": alloc(nx),
+           "lc": alloc(nlc) if need_lc else None,
+           "g_trans": torch.empty(B, sizes[0], sizes[-1], dtype=C128, device=device)}
+    out["lesser"] = (s_in, alloc(nsq), alloc(nsq), alloc(nx), alloc(nx)) if s_in is not None else None
+    out["greater"] = ((s_out, alloc(nsq), alloc(nsq), alloc(nx), alloc(nx))
+                      if s_out is not None else None)
+    status = torch.zeros(B, dtype=torch.int32, device=device)
+    nmax = max(sizes)
+    work = torch.empty(5 * B * nmax * nmax, dtype=C128, device=device)
+    args = [nat.stream_ptr(device), K, nat.host_i32(sizes), nat.ptr(arena), B, nat.ptr(z),
+            nat.ptr(sigL), nL, nat.ptr(sigR), nR, int(ortho),
+            nat.ptr(out["gl"]), nat.ptr(out["grd"]), nat.ptr(out["grl"]), nat.ptr(out["gru"]),
+            nat.ptr(out["lc"]), nat.ptr(out["g_trans"])]
+    for part in (out["lesser"], out["greater"]):
+        args += [nat.ptr(t) for t in part] if part else [nat.ptr(None)] * 5
+    args += [nat.ptr(work), nat.ptr(status)]
+    nat.call("dpnegf_rgf_full", *args)
+    if int(status.max().item()) & nat.STATUS_SINGULAR:
+        raise torch.linalg.LinAlgError("singular block in the recursive Green's function sweep")
+    return out
 
 
 class BTDSystems:

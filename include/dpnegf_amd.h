@@ -109,6 +109,49 @@ int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena
                     const void* s_out, void* gip, void* gpd, void* gpl, void* gpu, void* work,
                     int32_t* status);
 
+/* Complex elements of the work buffer dpnegf_rgf_transmission_grad needs. */
+int64_t dpnegf_rgf_grad_workspace_elems(int K, const int32_t* h_bsz, int B);
+
+/* Adjoint of the transmission: for the objective J = sum_b w[b] T(E_b),
+ * grad_X = dJ/dRe(X) + i dJ/dIm(X) for every on-site block hd_q and coupling
+ * block hu_q, hl_q (torch's complex gradient convention), summed over the B
+ * energies.  New: the reference obtains this only by torch autograd through
+ * recursive_gf (dpnegf/negf/recursive_green_cal.py:89-112).  Uses
+ *   dT = 2 Re Tr[W dH],  W = G Gamma_R G^H Gamma_L G,
+ *   W_ab = G_{a,K-1} C G_{0,b},  C = Gamma_R G_{0,K-1}^H Gamma_L,
+ * with the last block column from the backward sweep and the first block row
+ * G_{0,b} = (g_0 U_0 ... g_{b-1} U_{b-1}) G_bb.
+ *   grad_hd [sum n_q^2], grad_hu [sum n_q n_{q+1}], grad_hl [sum n_{q+1} n_q]
+ * Block sizes up to DPNEGF_FULL_NMAX; B <= 65535 per call. */
+int dpnegf_rgf_transmission_grad(void* stream, int K, const int32_t* h_bsz, const void* arena,
+                                 int B, const void* z, const void* sigL, int nL, const void* sigR,
+                                 int nR, int ortho, const double* w, void* grad_hd, void* grad_hu,
+                                 void* grad_hl, void* work, int32_t* status);
+
+/* out[e] = (accumulate ? out[e] : 0) + sum_b Re(c[b] * in[b * count + e]):
+ * the pole / quadrature sum of the Ozaki density, Ozaki.integrate
+ * (dpnegf/negf/density.py:171-181, 189-195), over the B points of a batch.
+ * out: double [count]; in: complex [B x count]; c: complex [B]. */
+int dpnegf_weighted_re_sum(void* stream, double* out, const void* in, const void* c,
+                           int64_t count, int B, int accumulate);
+
+/* C[b] = alpha op(A[b]) op(B[b]) + beta C[b], complex128, row-major with
+ * leading dimensions; op 0 = N, 1 = conjugate transpose, 2 = transpose,
+ * 3 = conjugate; a stride of 0 shares one matrix across the batch.  The
+ * products of DeviceProperty._cal_tc_ / _cal_dos_ / _cal_ldos_
+ * (dpnegf/negf/device_property.py:404, 428-438, 451-461) and of the density
+ * integrand (dpnegf/negf/density.py:193) run through it.  B <= 65535. */
+int dpnegf_zgemm_batched(void* stream, int B, int m, int n, int k, const void* A, int lda,
+                         int64_t strideA, int opA, const void* Bm, int ldb, int64_t strideB, int opB,
+                         double alpha_re, double alpha_im, double beta_re, double beta_im, void* C,
+                         int ldc, int64_t strideC);
+
+/* out[b][i] (+)= sum_j A[b][i][j] B[b][j][i]: the diagonal of A B (A m x n,
+ * B n x m, contiguous rows), for traces and per-orbital spectra
+ * (dpnegf/negf/device_property.py:404, 438, 461). */
+int dpnegf_diag_product(void* stream, void* out, const void* A, int64_t strideA, const void* Bm,
+                        int64_t strideB, int m, int n, int B, int accumulate);
+
 #ifdef __cplusplus
 }
 #endif

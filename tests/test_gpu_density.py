@@ -1,0 +1,88 @@
+"""GPU Ozaki density (all diagonal blocks, equilibrium + bias window) vs the
+oracle: Ozaki poles, oracle Lopez-Sancho self-energies and oracle RGF blocks,
+summed with the formula of ``density.py:164-195`` (pinned in
+test_oracle_ozaki_density.py).  Bar: 1e-10 relative, FP64.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import lead as olead, ozaki as ooz, rgf as orgf, surface_green as osg
+from dpnegf_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch.device("cuda")
+
+
+def _oracle_density(s, kBT, mu_L, mu, R, M, E_ref, eta_lead, eta_dev, vL, vR, n_gauss, fL, fR):
+    poles, res = ooz.ozaki_residues(M)
+
+    def sig(e, eta):
+        return [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e), hDL=l[2], sDL=l[5],
+                               etaLead=eta, E_ref=E_ref)[0] for l in (s.lead_L, s.lead_R)]
+
+    def rgf(e, eta_l, eta_d, lc=False):
+        sL, sR = sig(e, eta_l)
+        ans = orgf.recursive_gf(torch.tensor(e, dtype=torch.complex128), s.hl, s.hd, s.hu, s.sd,
+                                s.su, s.sl, sL, sR, eta=eta_d, E_ref=E_ref, need_gr_lc=lc)
+        return ans, sR
+
+    g0 = rgf(1j * R - mu, 1e-5, 0.0)[0][2]
+    dm = [1j * R * g for g in g0]
+    for p in range(len(poles)):
+        e = 1j * float(poles[p]) * kBT + mu_L - mu
+        g = rgf(e, eta_lead, eta_dev)[0][2]
+        for q in range(len(dm)):
+            dm[q] = dm[q] - ((-4 * 1j * kBT) * g[q] * float(res[p])).imag
+    dm = [d.real for d in dm]
+    xl, xu = min(vL, vR) - 8 * kBT, max(vL, vR) + 8 * kBT
+    xs, wl_ = ooz.gauss_xw(torch.scalar_tensor(xl, dtype=torch.float64),
+                           torch.scalar_tensor(xu, dtype=torch.float64), n=n_gauss)
+    neq = [0.0] * len(dm)
+    nK = s.sizes[-1]
+    for i in range(n_gauss):
+        e = float(xs[i])
+        ans, sR = rgf(e, eta_lead, eta_dev, lc=True)
+        gam = olead.sigma2gamma(sR)
+        gpad = torch.zeros(nK, nK, dtype=torch.complex128)
+        r = gam.shape[-1]
+        gpad[nK - r:, nK - r:] = gam
+        f = fR(torch.tensor(e + mu)) - fL(torch.tensor(e + mu))
+        for q in range(len(dm)):
+            col = ans[1][q]
+            neq[q] = neq[q] + float(wl_[i]) * (col @ gpad @ col.mH).real * f
+    return dm, neq
+
+
+def test_ozaki_density_matches_oracle():
+    from dpnegf_amd.negf.density import Ozaki, ozaki_density
+    dev = _dev()
+    s = wl.two_orbital_chain(5)
+    kBT = 0.025852
+    E_ref, mu_L, mu, R, M = -0.2, 0.05, 0.0, 1e6, 40
+    vL, vR, n_gauss = 0.05, -0.05, 8
+    fL = lambda x: 1 / (1 + torch.exp((x - (mu - vL)) / kBT))
+    fR = lambda x: 1 / (1 + torch.exp((x - (mu - vR)) / kBT))
+    ref_eq, ref_neq = _oracle_density(s, kBT, mu_L, mu, R, M, E_ref, 1e-5, 1e-6, vL, vR, n_gauss,
+                                      fL, fR)
+    sd = s.to(dev)
+    oz = Ozaki(R, M, n_gauss)
+    fLd = lambda x: 1 / (1 + torch.exp((x - (mu - vL)) / kBT))
+    eq, neq = ozaki_density((sd.hd, sd.hu, sd.hl, sd.sd, sd.su, sd.sl), sd.lead_L, sd.lead_R, kBT,
+                            mu_L, mu, R, oz.poles, oz.residues, E_ref, 1e-5, 1e-6, vL, vR, n_gauss,
+                            fLd, fR)
+    scale = max(r.abs().max().item() for r in ref_eq)
+    for q in range(len(eq)):
+        err = (eq[q].cpu() - ref_eq[q].reshape(eq[q].shape)).abs().max().item() / scale
+        assert err < 1e-10, (q, err)
+    scale = max(float(np.abs(np.asarray(r)).max()) for r in ref_neq)
+    for q in range(len(neq)):
+        err = (neq[q].cpu() - torch.as_tensor(ref_neq[q])).abs().max().item() / scale
+        assert err < 1e-10, (q, err)
