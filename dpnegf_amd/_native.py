@@ -39,6 +39,7 @@ _SIGS = {
     "dpnegf_lead_self_energy": ([_vp, _i32, _i32, _i32, _vp, _i64, _i32, _vp, _i32, _vp, _f64, _f64,
                                  _vp, _vp, _vp, _i32, _vp, _vp, _vp], _i32),
     "dpnegf_rgf_full_workspace_elems": ([_i32, _vp, _i32], _i64),
+    "dpnegf_lead_self_energy_workspace_elems": ([_i32, _i32, _i32], _i64),
     "dpnegf_rgf_full": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _i32] + [_vp] * 17,
                         _i32),
     "dpnegf_rgf_grad_workspace_elems": ([_i32, _vp, _i32], _i64),
@@ -48,6 +49,9 @@ _SIGS = {
     "dpnegf_zgemm_batched": ([_vp, _i32, _i32, _i32, _i32, _vp, _i32, _i64, _i32, _vp, _i32, _i64,
                               _i32, _f64, _f64, _f64, _f64, _vp, _i32, _i64], _i32),
     "dpnegf_diag_product": ([_vp, _vp, _vp, _i64, _vp, _i64, _i32, _i32, _i32, _i32], _i32),
+    "dpnegf_rgf_transmission_blocked_workspace_elems": ([_i32, _vp, _i32], _i64),
+    "dpnegf_rgf_transmission_blocked": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _i32,
+                                         _vp, _vp, _vp], _i32),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -110,9 +110,9 @@ int dpnegf_lead_self_energy(void* stream, int n, int nred, int mode, const void*
                             void* sigma, int ortho, int32_t* iters, int32_t* status,
                             double* resid) {
   if (n < 1 || nred < 1) return fail(DPNEGF_ERR_ARG, "bad lead sizes n=%d nred=%d", n, nred);
-  if (n > DPNEGF_LEAD_NMAX || nred > DPNEGF_LEAD_NMAX)
+  if (n > DPNEGF_FULL_NMAX || nred > DPNEGF_FULL_NMAX)
     return fail(DPNEGF_ERR_UNSUPPORTED, "lead size %d / coupling rows %d exceed %d", n, nred,
-                DPNEGF_LEAD_NMAX);
+                DPNEGF_FULL_NMAX);
   if (mode < 0 || mode > 2) return fail(DPNEGF_ERR_ARG, "mode must be 0, 1 or 2");
   if (mode != 0 && nred != n) return fail(DPNEGF_ERR_ARG, "modes 1/2 need nred == n");
   if (P < 0) return fail(DPNEGF_ERR_ARG, "P must be >= 0");
@@ -127,9 +127,22 @@ int dpnegf_lead_self_energy(void* stream, int n, int nred, int mode, const void*
   a.lead_stride = lead_stride; a.sys_idx = sys_idx; a.P = P; a.e = (const cx<double>*)e;
   a.eta = eta; a.eps = (cx<double>*)eps_ws; a.gs = (cx<double>*)gs; a.sigma = (cx<double>*)sigma;
   a.iters = iters; a.status = status; a.resid = resid; a.stop = stop;
-  hipError_t err = negf::launch_lead_self_energy(a, ortho != 0, (hipStream_t)stream);
+  hipError_t err;
+  if (n <= DPNEGF_LEAD_NMAX && nred <= DPNEGF_LEAD_NMAX) {
+    err = negf::launch_lead_self_energy(a, ortho != 0, (hipStream_t)stream);
+  } else {
+    if (P > 65535) return fail(DPNEGF_ERR_ARG, "P=%d exceeds 65535 per call for blocks > %d", P,
+                               DPNEGF_LEAD_NMAX);
+    err = negf::lead_ls_batched(a, (cx<double>*)eps_ws, (hipStream_t)stream);
+  }
   if (err != hipSuccess) return hip_fail(err, "lead_self_energy launch");
   return 0;
+}
+
+int64_t dpnegf_lead_self_energy_workspace_elems(int n, int nred, int P) {
+  if (n < 1 || nred < 1 || P < 0) return -1;
+  if (n <= DPNEGF_LEAD_NMAX && nred <= DPNEGF_LEAD_NMAX) return (int64_t)P * n * n;
+  return negf::lead_ls_batched_workspace(n, nred, P);
 }
 
 int64_t dpnegf_rgf_full_workspace_elems(int K, const int32_t* h_bsz, int B) {
@@ -243,6 +256,33 @@ int dpnegf_diag_product(void* stream, void* out, const void* A, int64_t strideA,
                                     (const cx<double>*)Bm, strideB, m, n, B, accumulate,
                                     (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "diag_product");
+  return 0;
+}
+
+int64_t dpnegf_rgf_transmission_blocked_workspace_elems(int K, const int32_t* h_bsz, int B) {
+  if (!h_bsz || K < 1 || B < 0) return -1;
+  return negf::rgf_transmission_ops_workspace(K, h_bsz, B);
+}
+
+int dpnegf_rgf_transmission_blocked(void* stream, int K, const int32_t* h_bsz, const void* arena,
+                                    int B, const void* z, const void* sigL, int nL,
+                                    const void* sigR, int nR, int ortho, double* T, void* work,
+                                    int32_t* status) {
+  int nmax = 0;
+  if (int r = check_blocks(K, h_bsz, DPNEGF_FULL_NMAX, &nmax)) return r;
+  if (B < 0) return fail(DPNEGF_ERR_ARG, "B must be >= 0");
+  if (B == 0) return 0;
+  if (B > 65535) return fail(DPNEGF_ERR_ARG, "B=%d exceeds 65535 per call", B);
+  if (!arena || !z || !sigL || !sigR || !T || !work) return fail(DPNEGF_ERR_ARG, "null device pointer");
+  if (nL < 1 || nL > h_bsz[0] || nR < 1 || nR > h_bsz[K - 1])
+    return fail(DPNEGF_ERR_ARG, "self-energy larger than its corner block");
+  negf::RgfOpsArgs a;
+  a.K = K; a.B = B; a.ortho = ortho; a.bsz = h_bsz; a.arena = (const cx<double>*)arena;
+  a.z = (const cx<double>*)z; a.sigL = (const cx<double>*)sigL; a.nL = nL;
+  a.sigR = (const cx<double>*)sigR; a.nR = nR; a.T = T; a.work = (cx<double>*)work;
+  a.status = status;
+  hipError_t e = negf::rgf_transmission_ops(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "rgf_transmission_blocked");
   return 0;
 }
 

@@ -49,19 +49,23 @@ __device__ __forceinline__ cx<R> cvt(cx<S> a) { return {R(a.re), R(a.im)}; }
 
 // ---- matrix-core wrappers: D(16x16) += A(16x4) * B(4x16) ------------------------
 // Operand layout (both precisions): lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; the accumulator holds D[4*(l/16)+r][l%16] in element r.
+// B[l/16][l%16].  Accumulator element r of lane l holds D[row(l/16, r)][l%16]:
+// rows interleaved (l/16 + 4r) for the f64 instruction (measured on gfx950),
+// consecutive (4(l/16) + r) for f32.
 template <typename R> struct mfma;
 template <> struct mfma<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
 };
 template <> struct mfma<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
 
 // ---- wave / workgroup reductions -------------------------------------------------

@@ -185,12 +185,169 @@ hipError_t zgemm_batched(int B, int m, int n, int k, const cx<double>* A, int ld
               beta, C, sc, ldc, nullptr);
 }
 
+// T[b] = Re sum_{i,j} A[b][i][j] B[b][j][i]   (A: m x n, B: n x m), one workgroup per b
+__global__ __launch_bounds__(256) void btrace_re_kernel(double* T, const cx<double>* A, int64_t sa,
+                                                        const cx<double>* Bm, int64_t sb, int m,
+                                                        int n) {
+  __shared__ double red[8];
+  const int b = blockIdx.x;
+  const cx<double>* a = A + (int64_t)b * sa;
+  const cx<double>* bb = Bm + (int64_t)b * sb;
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < m * n; e += blockDim.x) {
+    const int i = e / n, j = e % n;
+    const cx<double> x = a[e], y = bb[(int64_t)j * m + i];
+    acc += x.re * y.re - x.im * y.im;
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) T[b] = acc;
+}
+
 hipError_t weighted_re_sum(double* out, const cx<double>* in, const cx<double>* c, int64_t count,
                            int B, int accumulate, hipStream_t st) {
   const int grid = (int)std::min<int64_t>((count + 255) / 256, 65535);
   hipLaunchKernelGGL(bweighted_re_sum_kernel, dim3(grid), dim3(256), 0, st, out, in, c, count, B,
                      accumulate);
   return hipGetLastError();
+}
+
+
+// Lopez-Sancho helpers for the batched (large-block) path -----------------------
+// alpha0 = h10 - z s10, beta0 = h01 - z s01, eps = eps_s = H, per point
+__global__ void ls_init_kernel(cx<double>* al, cx<double>* be, cx<double>* eps, cx<double>* epss,
+                               const cx<double>* lead, int64_t lstride, const int32_t* sys,
+                               const cx<double>* z, int n, int B, double eta) {
+  const int64_t tot = (int64_t)B * n * n;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < tot;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(x / ((int64_t)n * n));
+    const int e = (int)(x % ((int64_t)n * n));
+    const int i = e / n, k = e % n;
+    const cx<double>* H = lead + (sys ? (int64_t)sys[b] * lstride : 0);
+    const cx<double>* h01 = H + n * n;
+    const cx<double>* s01 = H + 3 * n * n;
+    const cx<double> zz{z[b].re, z[b].im + eta};
+    be[x] = h01[e] - zz * s01[e];
+    al[x] = conj(h01[k * n + i]) - zz * conj(s01[k * n + i]);
+    eps[x] = H[e];
+    epss[x] = H[e];
+  }
+}
+
+// W = z S - X  (X = eps or eps_s), per point
+__global__ void ls_zs_minus_kernel(cx<double>* W, const cx<double>* X, const cx<double>* lead,
+                                   int64_t lstride, const int32_t* sys, const cx<double>* z,
+                                   int n, int B, double eta) {
+  const int64_t tot = (int64_t)B * n * n;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < tot;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(x / ((int64_t)n * n));
+    const int e = (int)(x % ((int64_t)n * n));
+    const cx<double>* S = lead + (sys ? (int64_t)sys[b] * lstride : 0) + 2 * n * n;
+    const cx<double> zz{z[b].re, z[b].im + eta};
+    W[x] = zz * S[e] - X[x];
+  }
+}
+
+// per-point max_ij |alpha_ij| + |beta_ij|; records the first converged iteration
+__global__ __launch_bounds__(256) void ls_conv_kernel(const cx<double>* al, const cx<double>* be,
+                                                      int n, int32_t* iters, int32_t* done,
+                                                      int32_t* remaining, int it, double stop) {
+  __shared__ double red[8];
+  const int b = blockIdx.x;
+  const int64_t off = (int64_t)b * n * n;
+  double m = 0.0;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x)
+    m = fmax(m, cabs(al[off + e]) + cabs(be[off + e]));
+  m = block_max(m, red);
+  if (threadIdx.x == 0 && !done[b]) {
+    if (m < stop) {
+      done[b] = 1;
+      iters[b] = it;
+    } else {
+      atomicAdd(remaining, 1);
+    }
+  }
+}
+
+__global__ void fill_i32_kernel(int32_t* p, int32_t v, int64_t count) {
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < count;
+       x += (int64_t)gridDim.x * blockDim.x)
+    p[x] = v;
+}
+
+// points still running after the iteration limit: flag and record the count
+__global__ void ls_mark_kernel(const int32_t* done, int32_t* status, int32_t* iters, int B, int it) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < B && !done[b]) {
+    if (status) status[b] |= STATUS_LS_NOCONV;
+    iters[b] = it;
+  }
+}
+
+// out = a - b elementwise
+__global__ void bsub_kernel(cx<double>* out, const cx<double>* a, const cx<double>* b, int64_t count) {
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < count;
+       x += (int64_t)gridDim.x * blockDim.x)
+    out[x] = a[x] - b[x];
+}
+
+// residual test of surface_green.py:211-220 from R = test g_s - I (per point)
+__global__ __launch_bounds__(256) void ls_resid_kernel(const cx<double>* Rm, int n, double* resid,
+                                                       int32_t* status) {
+  __shared__ double red[8];
+  const int b = blockIdx.x;
+  const int64_t off = (int64_t)b * n * n;
+  double m = 0.0;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    cx<double> v = Rm[off + e];
+    if (e / n == e % n) v.re -= 1.0;
+    m = fmax(m, cabs(v));
+  }
+  m = block_max(m, red);
+  if (threadIdx.x == 0) {
+    if (resid) resid[b] = m;
+    if (status) {
+      if (m >= 3.0e-5) atomicOr(&status[b], STATUS_LS_RESID_FAIL);
+      else if (m > 1.0e-8) atomicOr(&status[b], STATUS_LS_RESID_WARN);
+    }
+  }
+}
+
+// test = z S - H - X2 (per point)
+__global__ void ls_test_kernel(cx<double>* T, const cx<double>* X2, const cx<double>* lead,
+                               int64_t lstride, const int32_t* sys, const cx<double>* z, int n,
+                               int B, double eta) {
+  const int64_t tot = (int64_t)B * n * n;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < tot;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(x / ((int64_t)n * n));
+    const int e = (int)(x % ((int64_t)n * n));
+    const cx<double>* H = lead + (sys ? (int64_t)sys[b] * lstride : 0);
+    const cx<double> zz{z[b].re, z[b].im + eta};
+    T[x] = zz * H[2 * n * n + e] - H[e] - X2[x];
+  }
+}
+
+// coupling operands per point: out = z*s - h (op N) or z*conj(s)^T - conj(h)^T (op H, z not
+// conjugated), with h, s of shape r x c from the lead record at element offset `off`
+__global__ void ls_coupling_kernel(cx<double>* out, const cx<double>* lead, int64_t lstride,
+                                   const int32_t* sys, const cx<double>* z, double eta,
+                                   int64_t hoff, int64_t soff, int r, int c, int transpose, int B) {
+  const int64_t tot = (int64_t)B * r * c;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < tot;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(x / ((int64_t)r * c));
+    const int e = (int)(x % ((int64_t)r * c));
+    const cx<double>* base = lead + (sys ? (int64_t)sys[b] * lstride : 0);
+    const cx<double> zz{z[b].re, z[b].im + eta};
+    if (!transpose) {
+      out[x] = zz * base[soff + e] - base[hoff + e];
+    } else {  // element (i, k) of the c x r result = z conj(s[k][i]) - conj(h[k][i])
+      const int i = e / r, k = e % r;
+      out[x] = zz * conj(base[soff + (int64_t)k * c + i]) - conj(base[hoff + (int64_t)k * c + i]);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -404,6 +561,206 @@ hipError_t rgf_full(const RgfFullArgs& a, hipStream_t st) {
   return hipSuccess;
 }
 
+
+int64_t rgf_transmission_ops_workspace(int K, const int32_t* bsz, int B) {
+  int nmax = 0;
+  for (int q = 0; q < K; ++q) nmax = std::max(nmax, (int)bsz[q]);
+  return 8 * (int64_t)B * nmax * nmax;
+}
+
+// Transmission for any block size (<= GJ_NMAX_LARGE) by batched block operations:
+// the same sweep as the fused kernel (P = G^L_{0,q} carried forward), then
+// T = Re Tr[Gamma_L Q Gamma_R Q^H] with Q the corner of P.
+hipError_t rgf_transmission_ops(const RgfOpsArgs& a, hipStream_t st) {
+  const int K = a.K, B = a.B;
+  const std::vector<int> n(a.bsz, a.bsz + K);
+  int nmax = 0;
+  std::vector<int64_t> hoff(K);
+  int64_t sh = 0;
+  for (int q = 0; q < K; ++q) {
+    nmax = std::max(nmax, n[q]);
+    hoff[q] = sh;
+    sh += 2 * (int64_t)n[q] * n[q];
+    if (q + 1 < K) sh += 4 * (int64_t)n[q] * n[q + 1];
+  }
+  const int64_t m2 = (int64_t)B * nmax * nmax;
+  cx<double>* g = a.work;
+  cx<double>* Y = g + m2;
+  cx<double>* P = Y + m2;
+  cx<double>* P2 = P + m2;
+  cx<double>* gamL = P2 + m2;
+  cx<double>* gamR = gamL + m2;
+  cx<double>* V = gamR + m2;
+  cx<double>* W = V + m2;
+  const cx<double> one{1, 0}, mone{-1, 0}, zero{0, 0};
+  const cx<double>* H = a.arena;
+  auto hd = [&](int q) { return H + hoff[q]; };
+  auto sdp = [&](int q) { return H + hoff[q] + (int64_t)n[q] * n[q]; };
+  auto hu = [&](int q) { return H + hoff[q] + 2 * (int64_t)n[q] * n[q]; };
+  auto suu = [&](int q) { return hu(q) + (int64_t)n[q] * n[q + 1]; };
+  auto hl = [&](int q) { return suu(q) + (int64_t)n[q] * n[q + 1]; };
+  auto sl = [&](int q) { return hl(q) + (int64_t)n[q + 1] * n[q]; };
+  auto Uop = [&](int q) { return mat(hu(q), n[q + 1], 0, OP_N, a.ortho ? nullptr : suu(q), 0); };
+  auto Lop = [&](int q) { return mat(hl(q), n[q], 0, OP_N, a.ortho ? nullptr : sl(q), 0); };
+  auto grid_of = [](int64_t tot) { return (int)std::min<int64_t>((tot + 255) / 256, 65535); };
+  auto build = [&](int q, bool left, bool right) {
+    hipLaunchKernelGGL(bbuild_kernel, dim3(grid_of((int64_t)B * n[q] * n[q])), dim3(256), 0, st, g,
+                       hd(q), sdp(q), n[q], a.z, B, left ? a.sigL : nullptr, a.nL,
+                       right ? a.sigR : nullptr, a.nR);
+    return hipGetLastError();
+  };
+  const int n0 = n[0];
+  CHK(build(0, true, K == 1));
+  CHK(inverse(st, g, n0, B, a.status));
+  CHK(hipMemcpyAsync(P, g, sizeof(cx<double>) * B * n0 * n0, hipMemcpyDeviceToDevice, st));
+  for (int q = 0; q + 1 < K; ++q) {
+    const int nq = n[q], n1 = n[q + 1];
+    CHK(gemm(st, B, nq, n1, nq, mat(g, nq, (int64_t)nq * nq), Uop(q), one, zero, Y, (int64_t)nq * n1,
+             n1, a.z));
+    CHK(gemm(st, B, n0, n1, nq, mat(P, nq, (int64_t)n0 * nq), Uop(q), one, zero, P2,
+             (int64_t)n0 * n1, n1, a.z));
+    CHK(build(q + 1, false, q + 1 == K - 1));
+    CHK(gemm(st, B, n1, n1, nq, Lop(q), mat(Y, n1, (int64_t)nq * n1), mone, one, g, (int64_t)n1 * n1,
+             n1, a.z));
+    CHK(inverse(st, g, n1, B, a.status));
+    CHK(gemm(st, B, n0, n1, n1, mat(P2, n1, (int64_t)n0 * n1), mat(g, n1, (int64_t)n1 * n1), one,
+             zero, P, (int64_t)n0 * n1, n1, a.z));
+  }
+  const int nK = n[K - 1], nL = a.nL, nR = a.nR, c0 = nK - nR;
+  hipLaunchKernelGGL(bgamma_pad_kernel, dim3(grid_of((int64_t)B * nL * nL)), dim3(256), 0, st, gamL,
+                     a.sigL, nL, nL, B, 0, (const double*)nullptr, 1.0);
+  hipLaunchKernelGGL(bgamma_pad_kernel, dim3(grid_of((int64_t)B * nR * nR)), dim3(256), 0, st, gamR,
+                     a.sigR, nR, nR, B, 0, (const double*)nullptr, 1.0);
+  CHK(hipGetLastError());
+  // V = Q Gamma_R   [nL x nR];  W = V Q^H  [nL x nL];  T = Re Tr[Gamma_L W]
+  CHK(gemm(st, B, nL, nR, nR, mat(P + c0, nK, (int64_t)n0 * nK), mat(gamR, nR, (int64_t)nR * nR), one,
+           zero, V, (int64_t)nL * nR, nR, a.z));
+  CHK(gemm(st, B, nL, nL, nR, mat(V, nR, (int64_t)nL * nR), mat(P + c0, nK, (int64_t)n0 * nK, OP_H),
+           one, zero, W, (int64_t)nL * nL, nL, a.z));
+  hipLaunchKernelGGL(btrace_re_kernel, dim3(B), dim3(256), 0, st, a.T, gamL, (int64_t)nL * nL, W,
+                     (int64_t)nL * nL, nL, nL);
+  return hipGetLastError();
+}
+
+
+int64_t lead_ls_batched_workspace(int n, int nred, int B) {
+  const int m = std::max(n, nred);
+  return 9 * (int64_t)B * m * m + 4 * (int64_t)B;  // + int32 flags packed as complex slots
+}
+
+hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) {
+  const int n = a.n, nr = a.nred, B = a.P;
+  const int m = std::max(n, nr);
+  const int64_t m2 = (int64_t)B * m * m;
+  cx<double>* al = work;
+  cx<double>* be = al + m2;
+  cx<double>* eps = be + m2;
+  cx<double>* W = eps + m2;
+  cx<double>* Ai = W + m2;
+  cx<double>* Bi = Ai + m2;
+  cx<double>* Tm = Bi + m2;
+  cx<double>* X1 = Tm + m2;
+  cx<double>* X2 = X1 + m2;
+  int32_t* flags = (int32_t*)(X2 + m2);  // done [B], iteration counts [B], remaining [1]
+  int32_t* done = flags;
+  int32_t* iters = a.iters ? a.iters : flags + B;
+  int32_t* remaining = flags + 2 * B;
+  cx<double>* epss = a.gs;  // eps_s accumulates in the g_s output
+  const cx<double> one{1, 0}, zero{0, 0};
+  const int64_t nn = (int64_t)n * n;
+  auto grid_of = [](int64_t tot) { return (int)std::min<int64_t>((tot + 255) / 256, 65535); };
+  const int gridn = grid_of((int64_t)B * nn);
+  int32_t* status = a.status;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_of(2 * (int64_t)B + 1)), dim3(256), 0, st, flags, 0,
+                     2 * (int64_t)B + 1);
+  if (status) hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_of(B)), dim3(256), 0, st, status, 0, (int64_t)B);
+  CHK(hipGetLastError());
+  hipLaunchKernelGGL(ls_init_kernel, dim3(gridn), dim3(256), 0, st, al, be, eps, epss, a.lead,
+                     a.lead_stride, a.sys_idx, a.e, n, B, a.eta);
+  CHK(hipGetLastError());
+  auto M = [&](cx<double>* p) { return mat(p, n, nn); };
+  int it = 0;
+  bool converged = false;
+  while (true) {
+    ++it;
+    hipLaunchKernelGGL(ls_zs_minus_kernel, dim3(gridn), dim3(256), 0, st, W, eps, a.lead,
+                       a.lead_stride, a.sys_idx, a.e, n, B, a.eta);
+    CHK(hipGetLastError());
+    CHK(inverse(st, W, n, B, status));
+    CHK(gemm(st, B, n, n, n, M(al), M(W), one, zero, Ai, nn, n, nullptr));
+    CHK(gemm(st, B, n, n, n, M(be), M(W), one, zero, Bi, nn, n, nullptr));
+    CHK(gemm(st, B, n, n, n, M(Ai), M(be), one, one, eps, nn, n, nullptr));
+    CHK(gemm(st, B, n, n, n, M(Bi), M(al), one, one, eps, nn, n, nullptr));
+    CHK(gemm(st, B, n, n, n, M(Bi), M(al), one, one, epss, nn, n, nullptr));
+    CHK(gemm(st, B, n, n, n, M(Ai), M(al), one, zero, Tm, nn, n, nullptr));
+    std::swap(al, Tm);
+    CHK(gemm(st, B, n, n, n, M(Bi), M(be), one, zero, Tm, nn, n, nullptr));
+    std::swap(be, Tm);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, st, remaining, 0, (int64_t)1);
+    hipLaunchKernelGGL(ls_conv_kernel, dim3(B), dim3(256), 0, st, al, be, n, iters, done, remaining,
+                       it, a.stop);
+    CHK(hipGetLastError());
+    int32_t left = 0;
+    CHK(hipMemcpyAsync(&left, remaining, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CHK(hipStreamSynchronize(st));
+    if (left == 0) { converged = true; break; }
+    if (it >= 101) break;
+  }
+  if (!converged) {
+    // points still running at the iteration limit (surface_green.py:222-224)
+    hipLaunchKernelGGL(ls_mark_kernel, dim3((B + 255) / 256), dim3(256), 0, st, done, status, iters,
+                       B, it);
+    CHK(hipGetLastError());
+  }
+  // g_s = (z S - eps_s)^-1
+  hipLaunchKernelGGL(ls_zs_minus_kernel, dim3(gridn), dim3(256), 0, st, W, epss, a.lead,
+                     a.lead_stride, a.sys_idx, a.e, n, B, a.eta);
+  CHK(hipGetLastError());
+  CHK(inverse(st, W, n, B, status));
+  CHK(hipMemcpyAsync(a.gs, W, sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
+  const cx<double>* gs = a.gs;
+  // residual: X1 = (z s01 - h01) g_s ; X2 = X1 (z s10 - h10) ; R = (zS - H - X2) g_s
+  hipLaunchKernelGGL(ls_coupling_kernel, dim3(gridn), dim3(256), 0, st, Ai, a.lead, a.lead_stride,
+                     a.sys_idx, a.e, a.eta, nn, 3 * nn, n, n, 0, B);
+  hipLaunchKernelGGL(ls_coupling_kernel, dim3(gridn), dim3(256), 0, st, Bi, a.lead, a.lead_stride,
+                     a.sys_idx, a.e, a.eta, nn, 3 * nn, n, n, 1, B);
+  CHK(hipGetLastError());
+  CHK(gemm(st, B, n, n, n, M(Ai), mat(gs, n, nn), one, zero, X1, nn, n, nullptr));
+  CHK(gemm(st, B, n, n, n, M(X1), M(Bi), one, zero, X2, nn, n, nullptr));
+  hipLaunchKernelGGL(ls_test_kernel, dim3(gridn), dim3(256), 0, st, Tm, X2, a.lead, a.lead_stride,
+                     a.sys_idx, a.e, n, B, a.eta);
+  CHK(hipGetLastError());
+  CHK(gemm(st, B, n, n, n, M(Tm), mat(gs, n, nn), one, zero, X1, nn, n, nullptr));
+  hipLaunchKernelGGL(ls_resid_kernel, dim3(B), dim3(256), 0, st, X1, n, a.resid, status);
+  CHK(hipGetLastError());
+  // Sigma
+  const int64_t dl = 4 * nn, sdl = 4 * nn + (int64_t)nr * n;
+  if (a.mode == 0) {
+    // tau1 = e S_DL - H_DL [nr x n], tau2 = e S_DL^H - H_DL^H [n x nr]  (real-axis e: eta 0)
+    hipLaunchKernelGGL(ls_coupling_kernel, dim3(grid_of((int64_t)B * nr * n)), dim3(256), 0, st, Ai,
+                       a.lead, a.lead_stride, a.sys_idx, a.e, 0.0, dl, sdl, nr, n, 0, B);
+    hipLaunchKernelGGL(ls_coupling_kernel, dim3(grid_of((int64_t)B * nr * n)), dim3(256), 0, st, Bi,
+                       a.lead, a.lead_stride, a.sys_idx, a.e, 0.0, dl, sdl, nr, n, 1, B);
+    CHK(hipGetLastError());
+    CHK(gemm(st, B, n, nr, n, mat(gs, n, nn), mat(Bi, nr, (int64_t)n * nr), one, zero, X1,
+             (int64_t)n * nr, nr, nullptr));
+    CHK(gemm(st, B, nr, nr, n, mat(Ai, n, (int64_t)nr * n), mat(X1, nr, (int64_t)n * nr), one, zero,
+             a.sigma, (int64_t)nr * nr, nr, nullptr));
+  } else {
+    CHK(hipMemcpyAsync(W, gs, sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
+    CHK(inverse(st, W, n, B, status));
+    if (a.mode == 1) {
+      // Sigma = (e S - H) - g_s^-1 : the coupling kernel with h = H (offset 0), s = S (2nn)
+      hipLaunchKernelGGL(ls_coupling_kernel, dim3(gridn), dim3(256), 0, st, X1, a.lead,
+                         a.lead_stride, a.sys_idx, a.e, 0.0, (int64_t)0, 2 * nn, n, n, 0, B);
+      hipLaunchKernelGGL(bsub_kernel, dim3(gridn), dim3(256), 0, st, a.sigma, X1, W, (int64_t)B * nn);
+      CHK(hipGetLastError());
+    } else {
+      CHK(hipMemcpyAsync(a.sigma, W, sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
+    }
+  }
+  return hipSuccess;
+}
 
 int64_t rgf_grad_workspace(int K, const int32_t* bsz, int B) {
   int64_t nsq = 0, nx = 0, nlc = 0;

@@ -74,6 +74,23 @@ hipError_t zgemm_batched(int B, int m, int n, int k, const cx<double>* A, int ld
 hipError_t weighted_re_sum(double* out, const cx<double>* in, const cx<double>* c, int64_t count,
                            int B, int accumulate, hipStream_t st);
 int64_t rgf_grad_workspace(int K, const int32_t* bsz, int B);
+
+struct RgfOpsArgs {
+  int K, B, ortho;
+  const int32_t* bsz;          // host [K]
+  const cx<double>* arena;
+  const cx<double>* z;         // [B]
+  const cx<double>* sigL; int nL;
+  const cx<double>* sigR; int nR;
+  double* T;                   // [B]
+  cx<double>* work;            // rgf_transmission_ops_workspace elements
+  int32_t* status;
+};
+struct LeadArgs;
+int64_t lead_ls_batched_workspace(int n, int nred, int B);
+hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st);
+int64_t rgf_transmission_ops_workspace(int K, const int32_t* bsz, int B);
+hipError_t rgf_transmission_ops(const RgfOpsArgs& a, hipStream_t st);
 hipError_t rgf_grad(const RgfGradArgs& a, hipStream_t st);
 
 }  // namespace negf

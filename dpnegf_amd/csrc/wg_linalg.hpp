@@ -55,10 +55,10 @@ __device__ __forceinline__ void wg_gemm(int m, int n, int kd, FA ga, FB gb, EPI 
     const int t = wave + NW * s;
     if (t < nt) {
       const int j = (t % tn) * 16 + li;
-      const int ib = (t / tn) * 16 + 4 * lk;
+      const int ib = (t / tn) * 16;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = ib + r;
+        const int i = ib + mfma<R>::row(lk, r);
         if (i < m && j < n) epi(i, j, cx<R>{cr[s][r], ci[s][r]});
       }
     }

@@ -182,9 +182,39 @@ def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", sta
     dev = systems.device
     P = z.shape[0]
     T = out if out is not None else torch.empty(P, dtype=torch.float64, device=dev)
-    prec = {"fp64": nat.DPNEGF_FP64, "mixed": nat.DPNEGF_MIXED}[precision]
-    nat.call("dpnegf_rgf_transmission", nat.stream_ptr(dev), prec, len(systems.sizes),
-             systems._h_bsz, nat.ptr(systems.bsz), nat.ptr(systems.arena), systems.stride,
-             systems.nsys, nat.ptr(sys_idx), P, nat.ptr(z), nat.ptr(sigL), sigL.shape[-1],
-             nat.ptr(sigR), sigR.shape[-1], int(systems.ortho), nat.ptr(T), nat.ptr(status))
+    if max(systems.sizes) <= nat.TRANSMISSION_NMAX:
+        prec = {"fp64": nat.DPNEGF_FP64, "mixed": nat.DPNEGF_MIXED}[precision]
+        nat.call("dpnegf_rgf_transmission", nat.stream_ptr(dev), prec, len(systems.sizes),
+                 systems._h_bsz, nat.ptr(systems.bsz), nat.ptr(systems.arena), systems.stride,
+                 systems.nsys, nat.ptr(sys_idx), P, nat.ptr(z), nat.ptr(sigL), sigL.shape[-1],
+                 nat.ptr(sigR), sigR.shape[-1], int(systems.ortho), nat.ptr(T), nat.ptr(status))
+        return T
+    # larger blocks: block-operation sweep, one call per system and chunk of points
+    if precision != "fp64":
+        raise NotImplementedError("mixed precision needs block sizes <= 64")
+    K = len(systems.sizes)
+    idx = (sys_idx.cpu() if sys_idx is not None else torch.zeros(P, dtype=torch.int32))
+    chunk = 4096
+    for s in range(systems.nsys):
+        pts = (idx == s).nonzero().reshape(-1)
+        if pts.numel() == 0:
+            continue
+        pts = pts.to(dev)
+        arena = systems.arena[s * systems.stride:(s + 1) * systems.stride]
+        for c0 in range(0, pts.numel(), chunk):
+            sel = pts[c0:c0 + chunk]
+            B = sel.numel()
+            zz = z[sel].contiguous()
+            sl_ = sigL[sel].contiguous()
+            sr_ = sigR[sel].contiguous()
+            tt = torch.empty(B, dtype=torch.float64, device=dev)
+            st = torch.zeros(B, dtype=torch.int32, device=dev)
+            nwork = nat.load().dpnegf_rgf_transmission_blocked_workspace_elems(K, systems._h_bsz, B)
+            work = torch.empty(nwork, dtype=C128, device=dev)
+            nat.call("dpnegf_rgf_transmission_blocked", nat.stream_ptr(dev), K, systems._h_bsz,
+                     nat.ptr(arena), B, nat.ptr(zz), nat.ptr(sl_), sl_.shape[-1], nat.ptr(sr_),
+                     sr_.shape[-1], int(systems.ortho), nat.ptr(tt), nat.ptr(work), nat.ptr(st))
+            T[sel] = tt
+            if status is not None:
+                status[sel] = st
     return T

@@ -82,7 +82,8 @@ def self_energy_batched(leads, energies, eta_lead, sys_idx=None, E_ref=0.0, Bulk
     ortho = all(_orthogonal(l[3], l[4]) for l in leads)
     if len(leads) > 1 and sys_idx is None:
         raise ValueError("sys_idx is required with more than one lead system")
-    eps = torch.empty(P, n, n, dtype=C128, device=device)
+    eps = torch.empty(nat.load().dpnegf_lead_self_energy_workspace_elems(n, a, P), dtype=C128,
+                      device=device)
     gs = torch.empty(P, n, n, dtype=C128, device=device)
     sig = torch.empty(P, a, a, dtype=C128, device=device)
     iters = torch.zeros(P, dtype=torch.int32, device=device)

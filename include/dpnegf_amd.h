@@ -78,15 +78,32 @@ int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h
  *   mode 0: Sigma = (e sDL - hDL) g_s (e sDL^H - hDL^H)      [nred x nred]
  *   mode 1: Sigma = (e sL - hL) - g_s^{-1};  mode 2 (Bulk): g_s^{-1}
  *   stop   = decimation threshold on max|alpha| + |beta| (reference: 1e-40)
- *   eps_ws = workspace [P x n x n] complex; gs out [P x n x n]; sigma out [P x nred x nred]
+ *   eps_ws = workspace of dpnegf_lead_self_energy_workspace_elems(n, nred, P) complex
+ *            elements; gs out [P x n x n]; sigma out [P x nred x nred]
  *   iters/status/resid per point (may be null); status bits LS_* replace the
  *   reference's RuntimeError / ArithmeticError / warning.
- * n, nred up to DPNEGF_LEAD_NMAX. */
+ * n, nred <= DPNEGF_LEAD_NMAX run one fused workgroup per point; larger leads
+ * (up to DPNEGF_FULL_NMAX, P <= 65535 per call) iterate batched block operations
+ * until every point has converged. */
+int64_t dpnegf_lead_self_energy_workspace_elems(int n, int nred, int P);
 int dpnegf_lead_self_energy(void* stream, int n, int nred, int mode, const void* lead,
                             int64_t lead_stride, int nsys, const int32_t* sys_idx, int P,
                             const void* e, double eta, double stop, void* eps_ws, void* gs,
                             void* sigma, int ortho, int32_t* iters, int32_t* status,
                             double* resid);
+
+/* Complex elements of the work buffer dpnegf_rgf_transmission_blocked needs. */
+int64_t dpnegf_rgf_transmission_blocked_workspace_elems(int K, const int32_t* h_bsz, int B);
+
+/* Transmission of B energies of one system for block sizes up to
+ * DPNEGF_FULL_NMAX (the fused kernel stops at 64): the same sweep driven block
+ * by block from the host with batched products and inverses.  Same result
+ * contract as dpnegf_rgf_transmission (recursive_green_cal.py:198-228 and
+ * device_property.py:374-406).  B <= 65535. */
+int dpnegf_rgf_transmission_blocked(void* stream, int K, const int32_t* h_bsz, const void* arena,
+                                    int B, const void* z, const void* sigL, int nL,
+                                    const void* sigR, int nR, int ortho, double* T, void* work,
+                                    int32_t* status);
 
 /* Complex elements of the work buffer dpnegf_rgf_full needs. */
 int64_t dpnegf_rgf_full_workspace_elems(int K, const int32_t* h_bsz, int B);
