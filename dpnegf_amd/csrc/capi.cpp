@@ -95,11 +95,13 @@ int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h
   a.sys_idx = sys_idx; a.P = P; a.z = (const cx<double>*)z;
   a.sigL = (const cx<double>*)sigL; a.nL = nL; a.sigR = (const cx<double>*)sigR; a.nR = nR;
   a.T = T; a.status = status;
+  bool uniform = true;
+  for (int q = 0; q < K; ++q) uniform = uniform && h_bsz[q] == nmax;
   hipError_t e;
   if (precision == DPNEGF_FP64)
-    e = negf::launch_rgf_transmission<double>(a, nmax, ortho != 0, (hipStream_t)stream);
+    e = negf::launch_rgf_transmission<double>(a, nmax, uniform, ortho != 0, (hipStream_t)stream);
   else
-    e = negf::launch_rgf_transmission_mixed(a, nmax, ortho != 0, (hipStream_t)stream);
+    e = negf::launch_rgf_transmission_mixed(a, nmax, uniform, ortho != 0, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "rgf_transmission launch");
   return 0;
 }

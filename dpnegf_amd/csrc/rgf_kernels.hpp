@@ -47,8 +47,10 @@ struct RgfArgs {
 };
 
 template <typename R>
-hipError_t launch_rgf_transmission(const RgfArgs& a, int nmax, bool ortho, hipStream_t st);
-hipError_t launch_rgf_transmission_mixed(const RgfArgs& a, int nmax, bool ortho, hipStream_t st);
+hipError_t launch_rgf_transmission(const RgfArgs& a, int nmax, bool uniform, bool ortho,
+                                   hipStream_t st);
+hipError_t launch_rgf_transmission_mixed(const RgfArgs& a, int nmax, bool uniform, bool ortho,
+                                         hipStream_t st);
 
 struct LeadArgs {
   int n;                     // principal-layer size

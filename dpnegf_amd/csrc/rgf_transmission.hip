@@ -21,7 +21,8 @@
 
 namespace negf {
 
-template <typename R, int NMAX, bool ORTHO>
+// FIXED: every block has exactly NMAX rows (inverse with the compile-time layout)
+template <typename R, int NMAX, bool ORTHO, bool FIXED>
 __global__ __launch_bounds__(256) void rgf_transmission_kernel(RgfArgs a) {
   constexpr int LD = NMAX + 1;
   constexpr int NT = (NMAX / 16) * (NMAX / 16);
@@ -30,6 +31,7 @@ __global__ __launch_bounds__(256) void rgf_transmission_kernel(RgfArgs a) {
   constexpr int MAXR = gj_maxr(NMAX, 256);
   __shared__ cx<R> PG[2 * NMAX * LD];
   __shared__ GJScratch<R, NMAX> gjs;
+  __shared__ GJBScratch<R, NMAX> gjb;
   __shared__ double red[8];
 
   const int pt = blockIdx.x;
@@ -66,9 +68,22 @@ __global__ __launch_bounds__(256) void rgf_transmission_kernel(RgfArgs a) {
 
   const cx<double>* hd0 = base;
   const cx<double>* sd0 = base + n0 * n0;
-  int sing = wg_gj_inverse<R, 256, NMAX, MAXR>(
-      n0, [&](int i, int k) { return cvt<R>(-Mval(hd0, sd0, n0, i, k, true, K == 1)); },
-      [&](int i, int k, cx<R> v) { g[i * LD + k] = v; }, &gjs);
+  // FIXED: blocked inverse of g in place; otherwise the register-resident one
+  auto inverse_g = [&](int n) -> int {
+    if constexpr (FIXED)
+      return wg_gj_inverse_blocked<R, 256, NMAX, LD>(
+          g, [&](int i, int k, cx<R> v) { g[i * LD + k] = v; }, &gjb);
+    else
+      return wg_gj_inverse<R, 256, NMAX, MAXR>(
+          n, [&](int i, int k) { return g[i * LD + k]; },
+          [&](int i, int k, cx<R> v) { g[i * LD + k] = v; }, &gjs);
+  };
+  for (int e = threadIdx.x; e < n0 * n0; e += 256) {
+    const int i = e / n0, k = e % n0;
+    g[i * LD + k] = cvt<R>(-Mval(hd0, sd0, n0, i, k, true, K == 1));
+  }
+  __syncthreads();
+  int sing = inverse_g(n0);
   __syncthreads();
   for (int e = threadIdx.x; e < n0 * n0; e += 256) {
     const int i = e / n0, k = e % n0;
@@ -111,9 +126,7 @@ __global__ __launch_bounds__(256) void rgf_transmission_kernel(RgfArgs a) {
           g[i * LD + j] = cvt<R>(-(cvt<double>(v) + Mval(hd1, sd1, n1, i, j, false, last)));
         });
     __syncthreads();
-    sing |= wg_gj_inverse<R, 256, NMAX, MAXR>(
-        n1, [&](int i, int k) { return g[i * LD + k]; },
-        [&](int i, int k, cx<R> v) { g[i * LD + k] = v; }, &gjs);
+    sing |= inverse_g(n1);
     __syncthreads();
     // P <- P g_{q+1}
     wg_gemm<R, 4, MAXT_SQ>(
@@ -147,29 +160,38 @@ __global__ __launch_bounds__(256) void rgf_transmission_kernel(RgfArgs a) {
   }
 }
 
-template <typename R, int NMAX>
+template <typename R, int NMAX, bool FIXED>
 static hipError_t launch_nmax(const RgfArgs& a, bool ortho, hipStream_t st) {
   dim3 grid(a.P), block(256);
   if (ortho)
-    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, true>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, true, FIXED>), grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, false>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, false, FIXED>), grid, block, 0, st, a);
   return hipGetLastError();
 }
 
+template <typename R, int NMAX>
+static hipError_t launch_sel(const RgfArgs& a, bool uniform, int nmax, bool ortho, hipStream_t st) {
+  if (uniform && nmax == NMAX) return launch_nmax<R, NMAX, true>(a, ortho, st);
+  return launch_nmax<R, NMAX, false>(a, ortho, st);
+}
+
+// uniform: every block has nmax rows
 template <typename R>
-hipError_t launch_rgf_transmission(const RgfArgs& a, int nmax, bool ortho, hipStream_t st) {
-  if (nmax <= 16) return launch_nmax<R, 16>(a, ortho, st);
-  if (nmax <= 32) return launch_nmax<R, 32>(a, ortho, st);
-  if (nmax <= 48) return launch_nmax<R, 48>(a, ortho, st);
-  if (nmax <= 64) return launch_nmax<R, 64>(a, ortho, st);
+hipError_t launch_rgf_transmission(const RgfArgs& a, int nmax, bool uniform, bool ortho,
+                                   hipStream_t st) {
+  if (nmax <= 16) return launch_sel<R, 16>(a, uniform, nmax, ortho, st);
+  if (nmax <= 32) return launch_sel<R, 32>(a, uniform, nmax, ortho, st);
+  if (nmax <= 48) return launch_sel<R, 48>(a, uniform, nmax, ortho, st);
+  if (nmax <= 64) return launch_sel<R, 64>(a, uniform, nmax, ortho, st);
   return hipErrorInvalidValue;
 }
 
-template hipError_t launch_rgf_transmission<double>(const RgfArgs&, int, bool, hipStream_t);
+template hipError_t launch_rgf_transmission<double>(const RgfArgs&, int, bool, bool, hipStream_t);
 
-hipError_t launch_rgf_transmission_mixed(const RgfArgs& a, int nmax, bool ortho, hipStream_t st) {
-  return launch_rgf_transmission<float>(a, nmax, ortho, st);
+hipError_t launch_rgf_transmission_mixed(const RgfArgs& a, int nmax, bool uniform, bool ortho,
+                                         hipStream_t st) {
+  return launch_rgf_transmission<float>(a, nmax, uniform, ortho, st);
 }
 
 }  // namespace negf

@@ -67,13 +67,30 @@ __device__ __forceinline__ void wg_gemm(int m, int n, int kd, FA ga, FB gb, EPI 
 
 template <typename R, int NMAX>
 struct GJScratch {
-  cx<R> col[2][NMAX];
-  cx<R> row[NMAX];
+  cx<R> col[2][NMAX];   // current pivot column, double-buffered across steps
+  cx<R> row[NMAX];      // current pivot row
+  R pval[2][NMAX];      // per-row-group partial pivot magnitude of the next column
+  int pidx[2][NMAX];    // ... and its row
   int piv[NMAX];
   int pinv[NMAX];
-  int used[NMAX];
   int singular;
 };
+
+// 1/d through one division: conj(d) / |d|^2 (block pivots are far from the
+// overflow range, so Smith's scaling is not needed here).
+template <typename R>
+__device__ __forceinline__ cx<R> recip_fast(cx<R> d) {
+  const R s = R(1) / (d.re * d.re + d.im * d.im);
+  return {d.re * s, -d.im * s};
+}
+
+// Pivot key: LAPACK izamax's |re| + |im|; NaN counts as 0 so that a row is
+// always selectable (the selection below never returns an out-of-range row).
+template <typename R>
+__device__ __forceinline__ R pivot_key(cx<R> a) {
+  const R m = abs1(a);
+  return (m == m) ? m : R(0);
+}
 
 // In-place style inverse: reads the matrix through load(i,k), writes the
 // inverse through store(i,k,v).  Both may address the same storage.  NT is
@@ -82,82 +99,127 @@ struct GJScratch {
 // nonzero (uniformly) if an exactly-zero pivot was met.
 //
 // Pivot choice follows LAPACK's izamax: largest |re| + |im| among the rows
-// not yet used, first index on ties.  After the sweep the registers hold Y
-// with A^{-1}[j][piv[j']] = Y[piv[j]][j'], undone while storing.
+// not yet used, first index on ties.  The owners of the next column publish
+// one (magnitude, row) partial per row group while they update it, so each
+// step reduces at most min(G, n) partials from LDS instead of shuffling.
+// After the sweep the registers hold Y with A^{-1}[j][piv[j']] = Y[piv[j]][j'],
+// undone while storing.
 template <typename R, int NT, int NMAX, int MAXR, class FL, class FS>
 __device__ __forceinline__ int wg_gj_inverse(int n, FL load, FS store, GJScratch<R, NMAX>* sh) {
+  static_assert(MAXR <= 64, "row mask is 64 bits");
   const int t = threadIdx.x;
-  const int lane = t & (WAVE - 1);
   const int G = NT / n;
   const int k = t % n, g = t / n;
   const bool act = g < G;
+  const int ng = G < n ? G : n;  // row groups that own rows
+  uint64_t used = 0;             // bit r: row g + G*r already a pivot row
   cx<R> v[MAXR];
 #pragma unroll
   for (int r = 0; r < MAXR; ++r) {
     const int i = g + G * r;
     v[r] = (act && i < n) ? load(i, k) : cx<R>{0, 0};
   }
-  for (int i = t; i < n; i += NT) sh->used[i] = 0;
   if (t == 0) sh->singular = 0;
+  // publish column 0 and its partial pivots
   if (act && k == 0) {
+    R best = R(-1);
+    int bi = n;
 #pragma unroll
     for (int r = 0; r < MAXR; ++r) {
       const int i = g + G * r;
-      if (i < n) sh->col[0][i] = v[r];
+      if (i < n) {
+        sh->col[0][i] = v[r];
+        const R m = pivot_key(v[r]);
+        if (m > best) { best = m; bi = i; }
+      }
     }
+    if (g < ng) { sh->pval[0][g] = best; sh->pidx[0][g] = bi; }
   }
   __syncthreads();
   for (int j = 0; j < n; ++j) {
-    const cx<R>* cb = sh->col[j & 1];
-    // pivot search, redundantly in every wave
+    const int b = j & 1;
     R best = R(-1);
     int bi = n;
-    for (int i = lane; i < n; i += WAVE) {
-      if (!sh->used[i]) {
-        const R m_ = abs1(cb[i]);
-        if (m_ > best) { best = m_; bi = i; }
-      }
+    for (int q = 0; q < ng; ++q) {
+      const R m = sh->pval[b][q];
+      const int i = sh->pidx[b][q];
+      if (m > best || (m == best && i < bi)) { best = m; bi = i; }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const R ob = __shfl_xor(best, o, WAVE);
-      const int oi = __shfl_xor(bi, o, WAVE);
-      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-    }
-    const int p = bi;
+    const int p = bi < n ? bi : n - 1;
+    if (act && p % G == g) used |= 1ull << (p / G);
     if (act) {
 #pragma unroll
       for (int r = 0; r < MAXR; ++r)
         if (g + G * r == p) sh->row[k] = v[r];
     }
     __syncthreads();
-    const cx<R> d = cb[p];
+    const cx<R> d = sh->col[b][p];
     if (t == 0) {
-      sh->used[p] = 1;
       sh->piv[j] = p;
       if (d.re == R(0) && d.im == R(0)) sh->singular = 1;
     }
-    const cx<R> dinv = recip(d);
+    const cx<R> dinv = recip_fast(d);
     const cx<R> rs = sh->row[k] * dinv;
-    cx<R>* nb = sh->col[(j + 1) & 1];
-    if (act) {
+    const cx<R>* cb = sh->col[b];
+    const bool colj = (k == j);
+    const cx<R> prow = colj ? dinv : rs;
+    if constexpr (NT <= 256 && MAXR <= 16) {
+      // every column read first (rows past n read a valid dummy), then selects
+      cx<R> f[MAXR];
+#pragma unroll
+      for (int r = 0; r < MAXR; ++r) {
+        const int i = g + G * r;
+        f[r] = cb[i < n ? i : 0];
+      }
+#pragma unroll
+      for (int r = 0; r < MAXR; ++r) {
+        const cx<R> upd = colj ? -(f[r] * dinv) : v[r] - f[r] * rs;
+        v[r] = (g + G * r == p) ? prow : upd;
+      }
+    } else {
+      // many rows per thread (1024-thread inverse): keep register pressure low
 #pragma unroll
       for (int r = 0; r < MAXR; ++r) {
         const int i = g + G * r;
         if (i < n) {
           if (i == p) {
-            v[r] = (k == j) ? dinv : rs;
+            v[r] = prow;
           } else {
             const cx<R> f = cb[i];
-            v[r] = (k == j) ? -(f * dinv) : v[r] - f * rs;
+            v[r] = colj ? -(f * dinv) : v[r] - f * rs;
           }
-          if (k == j + 1) nb[i] = v[r];
         }
       }
     }
+    if (act) {
+      const bool nxt = (k == j + 1);
+      R nbest = R(-1);
+      int nbi = n;
+#pragma unroll
+      for (int r = 0; r < MAXR; ++r) {
+        const int i = g + G * r;
+        if (i < n) {
+          if (nxt) {
+            sh->col[b ^ 1][i] = v[r];
+            if (!((used >> r) & 1)) {
+              const R m = pivot_key(v[r]);
+              if (m > nbest) { nbest = m; nbi = i; }
+            }
+          }
+        }
+      }
+      if (nxt && g < ng) { sh->pval[b ^ 1][g] = nbest; sh->pidx[b ^ 1][g] = nbi; }
+    }
     __syncthreads();
   }
-  for (int j = t; j < n; j += NT) sh->pinv[sh->piv[j]] = j;
+  // pinv starts as the identity so every store index stays in [0, n) even if a
+  // pivot row were ever selected twice
+  for (int j = t; j < n; j += NT) sh->pinv[j] = j;
+  __syncthreads();
+  for (int j = t; j < n; j += NT) {
+    const int pj = sh->piv[j];
+    if (pj >= 0 && pj < n) sh->pinv[pj] = j;
+  }
   __syncthreads();
   const int sing = sh->singular;
   if (act) {
@@ -168,6 +230,254 @@ __device__ __forceinline__ int wg_gj_inverse(int n, FL load, FS store, GJScratch
       if (i < n) store(sh->pinv[i], pc, v[r]);
     }
   }
+  __syncthreads();
+  return sing;
+}
+
+
+
+// Row groups of the fixed-size inverse: a power of two with N / G rows each.
+constexpr int gj_groups(int N, int NT) {
+  int g = 1;
+  while (2 * g * N <= NT && N % (2 * g) == 0) g *= 2;
+  return g;
+}
+
+// Same algorithm and pivot rule as wg_gj_inverse for a matrix of exactly N
+// rows, with the thread layout fixed at compile time: G = gj_groups(N, NT)
+// row groups of N / G rows, so every register slot holds a valid row, the
+// pivot owner is p % G (a mask) and the partials reduce in registers.
+template <typename R, int NT, int N, class FL, class FS>
+__device__ __forceinline__ int wg_gj_inverse_fixed(FL load, FS store, GJScratch<R, N>* sh) {
+  constexpr int G = gj_groups(N, NT);
+  constexpr int MAXR = N / G;
+  static_assert(G * MAXR == N && G * N <= NT && MAXR <= 64, "layout");
+  const int t = threadIdx.x;
+  const int k = t % N, g = t / N;
+  const bool act = t < G * N;
+  uint64_t used = 0;
+  cx<R> v[MAXR];
+#pragma unroll
+  for (int r = 0; r < MAXR; ++r) v[r] = act ? load(g + G * r, k) : cx<R>{0, 0};
+  if (t == 0) sh->singular = 0;
+  if (act && k == 0) {
+    R best = R(-1);
+    int bi = N;
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) {
+      sh->col[0][g + G * r] = v[r];
+      const R m = pivot_key(v[r]);
+      if (m > best) { best = m; bi = g + G * r; }
+    }
+    sh->pval[0][g] = best;
+    sh->pidx[0][g] = bi;
+  }
+  __syncthreads();
+  for (int j = 0; j < N; ++j) {
+    const int b = j & 1;
+    R pv[G];
+    int pi[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) { pv[q] = sh->pval[b][q]; pi[q] = sh->pidx[b][q]; }
+    R best = pv[0];
+    int bi = pi[0];
+#pragma unroll
+    for (int q = 1; q < G; ++q)
+      if (pv[q] > best || (pv[q] == best && pi[q] < bi)) { best = pv[q]; bi = pi[q]; }
+    const int p = bi < N ? bi : N - 1;
+    const bool own = act && ((p & (G - 1)) == g);
+    if (own) used |= 1ull << (p / G);  // power-of-two G: a shift
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < MAXR; ++r)
+        if (g + G * r == p) sh->row[k] = v[r];
+    }
+    __syncthreads();
+    const cx<R> d = sh->col[b][p];
+    if (t == 0) {
+      sh->piv[j] = p;
+      if (d.re == R(0) && d.im == R(0)) sh->singular = 1;
+    }
+    const cx<R> dinv = recip_fast(d);
+    const cx<R> rs = sh->row[k] * dinv;
+    const cx<R>* cb = sh->col[b];
+    const bool colj = (k == j);
+    // issue every column read before any arithmetic; updates are selects
+    cx<R> f[MAXR];
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) f[r] = cb[g + G * r];
+    const cx<R> prow = colj ? dinv : rs;
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) {
+      const cx<R> upd = colj ? -(f[r] * dinv) : v[r] - f[r] * rs;
+      v[r] = (g + G * r == p) ? prow : upd;
+    }
+    if (act && k == j + 1) {
+      R nbest = R(-1);
+      int nbi = N;
+#pragma unroll
+      for (int r = 0; r < MAXR; ++r) {
+        sh->col[b ^ 1][g + G * r] = v[r];
+        const R m = ((used >> r) & 1) ? R(-1) : pivot_key(v[r]);
+        if (m > nbest) { nbest = m; nbi = g + G * r; }
+      }
+      sh->pval[b ^ 1][g] = nbest;
+      sh->pidx[b ^ 1][g] = nbi;
+    }
+    __syncthreads();
+  }
+  for (int j = t; j < N; j += NT) sh->pinv[j] = j;
+  __syncthreads();
+  for (int j = t; j < N; j += NT) {
+    const int pj = sh->piv[j];
+    if (pj >= 0 && pj < N) sh->pinv[pj] = j;
+  }
+  __syncthreads();
+  const int sing = sh->singular;
+  if (act) {
+    const int pc = sh->piv[k];
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) store(sh->pinv[g + G * r], pc, v[r]);
+  }
+  __syncthreads();
+  return sing;
+}
+
+
+// ---- blocked Gauss-Jordan (matrix in LDS) -------------------------------------
+template <typename R, int N>
+struct GJBScratch {
+  cx<R> Z[4][N];  // pivot rows of the current panel, as they were before it
+  int pr[4];      // those rows
+  int piv[N];
+  int pinv[N];
+  int singular;
+};
+
+__device__ __forceinline__ double bcast_lane(double x, int src) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float bcast_lane(float x, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
+}
+
+// In-place inverse of the N x N matrix A (LDS, leading dimension LDA), N <= 64
+// and a multiple of 4, workgroup of NT threads.  Panels of 4 columns are swept
+// by wave 0 (one row per lane; pivot = largest |re| + |im| among unused rows,
+// compared in single precision, lowest row on ties; the pivot row is broadcast
+// with readlane), then the rest of the matrix takes the panel's rank-4 update
+// on the matrix cores:  A[:, C] <- A[:, C] (pivot rows zeroed) + Jnew Z,  with
+// Jnew the swept panel and Z the pivot rows before the panel.  As in
+// wg_gj_inverse, rows are never swapped; store(i, k, v) receives the inverse
+// in natural order.  Returns nonzero if an exactly-zero pivot was met.
+template <typename R, int NT, int N, int LDA, class FS>
+__device__ __forceinline__ int wg_gj_inverse_blocked(cx<R>* A, FS store, GJBScratch<R, N>* sh) {
+  constexpr int BW = 4;
+  constexpr int NW = NT / WAVE;
+  constexpr int NTILE = ((N + 15) / 16) * ((N + 15) / 16);
+  constexpr int MAXT = (NTILE + NW - 1) / NW;
+  static_assert(N <= WAVE && N % BW == 0, "blocked inverse: N <= 64, multiple of 4");
+  const int t = threadIdx.x;
+  const int lane = t & (WAVE - 1);
+  const int wave = t / WAVE;
+  if (t == 0) sh->singular = 0;
+  bool used = false;
+  for (int j0 = 0; j0 < N; j0 += BW) {
+    if (wave == 0) {
+      const bool row_ok = lane < N;
+      cx<R> v[BW];
+#pragma unroll
+      for (int c = 0; c < BW; ++c) v[c] = row_ok ? A[lane * LDA + j0 + c] : cx<R>{0, 0};
+#pragma unroll
+      for (int jj = 0; jj < BW; ++jj) {
+        unsigned long long key = 0;
+        if (row_ok && !used) {
+          const float kf = (float)pivot_key(v[jj]);
+          key = ((unsigned long long)(__float_as_uint(kf) + 1u) << 32) |
+                (unsigned long long)(unsigned int)(WAVE - 1 - lane);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          const unsigned int lo = __shfl_xor((unsigned int)key, o, WAVE);
+          const unsigned int hi = __shfl_xor((unsigned int)(key >> 32), o, WAVE);
+          const unsigned long long ok = ((unsigned long long)hi << 32) | lo;
+          key = ok > key ? ok : key;
+        }
+        int p = WAVE - 1 - (int)(key & 0xffffffffull);
+        p = __builtin_amdgcn_readfirstlane(p < N ? p : N - 1);
+        if (lane == p) used = true;
+        cx<R> prow[BW];
+#pragma unroll
+        for (int c = 0; c < BW; ++c) prow[c] = {bcast_lane(v[c].re, p), bcast_lane(v[c].im, p)};
+        const cx<R> d = prow[jj];
+        const cx<R> dinv = recip_fast(d);
+        if (lane == 0) {
+          sh->piv[j0 + jj] = p;
+          sh->pr[jj] = p;
+          if (d.re == R(0) && d.im == R(0)) sh->singular = 1;
+        }
+        const cx<R> fj = v[jj];
+        const bool isp = (lane == p);
+#pragma unroll
+        for (int c = 0; c < BW; ++c) {
+          if (c == jj) {
+            v[c] = isp ? dinv : -(fj * dinv);
+          } else {
+            const cx<R> rc = prow[c] * dinv;
+            v[c] = isp ? rc : v[c] - fj * rc;
+          }
+        }
+      }
+      if (row_ok) {
+#pragma unroll
+        for (int c = 0; c < BW; ++c) A[lane * LDA + j0 + c] = v[c];
+      }
+    }
+    __syncthreads();
+    for (int e = t; e < BW * N; e += NT) {
+      const int tt = e / N, c = e % N;
+      const bool inpanel = (c >= j0 && c < j0 + BW);
+      sh->Z[tt][c] = inpanel ? cx<R>{0, 0} : A[sh->pr[tt] * LDA + c];
+    }
+    __syncthreads();
+    for (int e = t; e < BW * N; e += NT) {
+      const int tt = e / N, c = e % N;
+      if (c < j0 || c >= j0 + BW) A[sh->pr[tt] * LDA + c] = cx<R>{0, 0};
+    }
+    __syncthreads();
+    wg_gemm<R, NW, MAXT>(
+        N, N, BW, [&](int i, int k) { return A[i * LDA + j0 + k]; },
+        [&](int k, int c) { return sh->Z[k][c]; },
+        [&](int i, int c, cx<R> val) {
+          if (c < j0 || c >= j0 + BW) A[i * LDA + c] = A[i * LDA + c] + val;
+        });
+    __syncthreads();
+  }
+  for (int j = t; j < N; j += NT) sh->pinv[j] = j;
+  __syncthreads();
+  for (int j = t; j < N; j += NT) {
+    const int pj = sh->piv[j];
+    if (pj >= 0 && pj < N) sh->pinv[pj] = j;
+  }
+  __syncthreads();
+  constexpr int PER = (N * N + NT - 1) / NT;
+  cx<R> val[PER];
+#pragma unroll
+  for (int s2 = 0; s2 < PER; ++s2) {
+    const int e = t + NT * s2;
+    val[s2] = (e < N * N) ? A[(e / N) * LDA + (e % N)] : cx<R>{0, 0};
+  }
+  __syncthreads();
+  const int sing = sh->singular;
+#pragma unroll
+  for (int s2 = 0; s2 < PER; ++s2) {
+    const int e = t + NT * s2;
+    if (e < N * N) store(sh->pinv[e / N], sh->piv[e % N], val[s2]);
+  }
+  __syncthreads();
   return sing;
 }
 

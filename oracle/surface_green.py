@@ -24,12 +24,15 @@ LS_RESID_WARN = 1.0e-8  # surface_green.py:216
 LS_MAX_ITER = 101       # surface_green.py:222
 
 
-def lopez_sancho(H, h01, S, s01, z, stop=LS_STOP):
+def lopez_sancho(H, h01, S, s01, z, stop=LS_STOP, variant="solve"):
     """Lopez-Sancho decimation; returns ``(g_s, iterations, residual)``.
 
     ``z`` already carries ``+ i*eta_lead``.  Raises ``ArithmeticError`` on a
     failed Dyson residual and ``RuntimeError`` past 100 iterations, exactly as
-    ``surface_green.py:206-224``.
+    ``surface_green.py:206-224``.  ``variant="solve"`` solves W X = [alpha|beta]
+    as the reference does; ``variant="inverse"`` forms W^{-1} and multiplies
+    (the association the GPU kernel uses) -- the two differ only by rounding,
+    which sets the scale of agreement an ill-conditioned point allows.
     """
     n = H.shape[0]
     h10 = h01.conj().T
@@ -42,9 +45,13 @@ def lopez_sancho(H, h01, S, s01, z, stop=LS_STOP):
     it = 0
     while True:
         it += 1
-        rhs = np.concatenate([alpha, beta], axis=1)
-        x = sla.solve(zS - eps, rhs, check_finite=False)
-        xa, xb = x[:, :n], x[:, n:]
+        if variant == "solve":
+            rhs = np.concatenate([alpha, beta], axis=1)
+            x = sla.solve(zS - eps, rhs, check_finite=False)
+            xa, xb = x[:, :n], x[:, n:]
+        else:
+            w = np.linalg.inv(zS - eps)
+            xa, xb = w @ alpha, w @ beta
         b_xa = beta @ xa
         eps = eps + alpha @ xb + b_xa
         eps_s = eps_s + b_xa

@@ -25,7 +25,8 @@ def _oracle_density(s, kBT, mu_L, mu, R, M, E_ref, eta_lead, eta_dev, vL, vR, n_
     poles, res = ooz.ozaki_residues(M)
 
     def sig(e, eta):
-        return [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e), hDL=l[2], sDL=l[5],
+        return [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e, dtype=torch.complex128),
+                               hDL=l[2], sDL=l[5],
                                etaLead=eta, E_ref=E_ref)[0] for l in (s.lead_L, s.lead_R)]
 
     def rgf(e, eta_l, eta_d, lc=False):
@@ -54,7 +55,8 @@ def _oracle_density(s, kBT, mu_L, mu, R, M, E_ref, eta_lead, eta_dev, vL, vR, n_
         gpad = torch.zeros(nK, nK, dtype=torch.complex128)
         r = gam.shape[-1]
         gpad[nK - r:, nK - r:] = gam
-        f = fR(torch.tensor(e + mu)) - fL(torch.tensor(e + mu))
+        ex = torch.tensor(e + mu, dtype=torch.float64)
+        f = fR(ex) - fL(ex)
         for q in range(len(dm)):
             col = ans[1][q]
             neq[q] = neq[q] + float(wl_[i]) * (col @ gpad @ col.mH).real * f

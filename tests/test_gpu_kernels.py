@@ -60,3 +60,23 @@ def test_shared_operand_and_reductions():
              m * n, B, 0)
     ref = (c.cpu().view(B, 1, 1) * A.cpu()).real.sum(0).reshape(-1)
     assert (w.cpu() - ref).abs().max() < 1e-12
+
+
+def test_nan_inputs_give_nan_not_a_fault():
+    """Regression: a NaN column used to leave the Gauss-Jordan pivot unset and
+    its permutation partly uninitialised; the selection now always picks an
+    unused row, so NaN inputs propagate as NaN results."""
+    from dpnegf_amd import workloads as wl
+    from dpnegf_amd.negf.recursive_green_cal import BTDSystems, recursive_gf, rgf_transmission
+    dev = _dev()
+    hd, sd, hl, hu, sl, su = wl.random_btd([6, 6, 6], seed=1)
+    to = lambda xs: [x.to(dev) for x in xs]
+    E = torch.tensor([0.1, float("nan")], dtype=torch.complex128, device=dev)
+    se = torch.zeros(2, 6, 6, dtype=torch.complex128, device=dev) - 0.3j * torch.eye(6, device=dev)
+    out = recursive_gf(E, to(hl), to(hd), to(hu), to(sd), to(su), to(sl), se, se, eta=1e-3)
+    torch.cuda.synchronize()
+    assert torch.isfinite(out[2][0][0]).all() and not torch.isfinite(out[2][0][1]).all()
+    bs = BTDSystems([(to(hd), to(hu), to(hl), to(sd), to(su), to(sl))], dev)
+    T = rgf_transmission(bs, (E + 1e-3j).contiguous(), se, se)
+    torch.cuda.synchronize()
+    assert torch.isfinite(T[0]) and torch.isnan(T[1])
