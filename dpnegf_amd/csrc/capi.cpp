@@ -143,7 +143,7 @@ int dpnegf_lead_self_energy(void* stream, int n, int nred, int mode, const void*
 
 int64_t dpnegf_lead_self_energy_workspace_elems(int n, int nred, int P) {
   if (n < 1 || nred < 1 || P < 0) return -1;
-  if (n <= DPNEGF_LEAD_NMAX && nred <= DPNEGF_LEAD_NMAX) return (int64_t)P * n * n;
+  if (n <= DPNEGF_LEAD_NMAX && nred <= DPNEGF_LEAD_NMAX) return negf::lead_self_energy_workspace(n, P);
   return negf::lead_ls_batched_workspace(n, nred, P);
 }
 

@@ -82,6 +82,29 @@ __device__ __forceinline__ T wave_max(T v) {
   return v;
 }
 
+// Maximum of a 32-bit unsigned value over a full wave (all 64 lanes active),
+// returned wave-uniform.  Four DPP steps leave every lane with the maximum of
+// its 16-lane row (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
+// row_mirror); the four row maxima are then combined through readlane.  No
+// LDS round trip, unlike __shfl_xor (ds_bpermute).
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp_max_step(unsigned v) {
+  const unsigned o = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+  return v > o ? v : o;
+}
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+  v = dpp_max_step<0xB1>(v);   // quad_perm [1,0,3,2]
+  v = dpp_max_step<0x4E>(v);   // quad_perm [2,3,0,1]
+  v = dpp_max_step<0x141>(v);  // row_half_mirror
+  v = dpp_max_step<0x140>(v);  // row_mirror
+  const unsigned a = (unsigned)__builtin_amdgcn_readlane((int)v, 0);
+  const unsigned b = (unsigned)__builtin_amdgcn_readlane((int)v, 16);
+  const unsigned c = (unsigned)__builtin_amdgcn_readlane((int)v, 32);
+  const unsigned d = (unsigned)__builtin_amdgcn_readlane((int)v, 48);
+  const unsigned ab = a > b ? a : b, cd = c > d ? c : d;
+  return ab > cd ? ab : cd;
+}
+
 // Sum over the workgroup; `red` is LDS scratch of >= nwaves entries.  All
 // threads get the result.
 template <typename T>

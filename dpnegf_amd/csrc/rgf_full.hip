@@ -666,7 +666,7 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
   int32_t* iters = a.iters ? a.iters : flags + B;
   int32_t* remaining = flags + 2 * B;
   cx<double>* epss = a.gs;  // eps_s accumulates in the g_s output
-  const cx<double> one{1, 0}, zero{0, 0};
+  const cx<double> one{1, 0}, zero{0, 0}, mone{-1, 0};
   const int64_t nn = (int64_t)n * n;
   auto grid_of = [](int64_t tot) { return (int)std::min<int64_t>((tot + 255) / 256, 65535); };
   const int gridn = grid_of((int64_t)B * nn);
@@ -686,15 +686,26 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
     hipLaunchKernelGGL(ls_zs_minus_kernel, dim3(gridn), dim3(256), 0, st, W, eps, a.lead,
                        a.lead_stride, a.sys_idx, a.e, n, B, a.eta);
     CHK(hipGetLastError());
+    // X_a = W^-1 alpha, X_b = W^-1 beta solved as the reference solves them (surface_green.py:189-194):
+    // through the inverse, refined once in FP64, X += W^-1 (rhs - W X).  Without the
+    // refinement, W^-1 times rhs loses the Dyson residual near band edges at small eta.
+    CHK(hipMemcpyAsync(X1, W, sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
     CHK(inverse(st, W, n, B, status));
-    CHK(gemm(st, B, n, n, n, M(al), M(W), one, zero, Ai, nn, n, nullptr));
-    CHK(gemm(st, B, n, n, n, M(be), M(W), one, zero, Bi, nn, n, nullptr));
-    CHK(gemm(st, B, n, n, n, M(Ai), M(be), one, one, eps, nn, n, nullptr));
-    CHK(gemm(st, B, n, n, n, M(Bi), M(al), one, one, eps, nn, n, nullptr));
-    CHK(gemm(st, B, n, n, n, M(Bi), M(al), one, one, epss, nn, n, nullptr));
-    CHK(gemm(st, B, n, n, n, M(Ai), M(al), one, zero, Tm, nn, n, nullptr));
+    cx<double>* rhs[2] = {al, be};
+    cx<double>* sol[2] = {Ai, Bi};
+    for (int q = 0; q < 2; ++q) {
+      CHK(gemm(st, B, n, n, n, M(W), M(rhs[q]), one, zero, sol[q], nn, n, nullptr));
+      CHK(hipMemcpyAsync(X2, rhs[q], sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
+      CHK(gemm(st, B, n, n, n, M(X1), M(sol[q]), mone, one, X2, nn, n, nullptr));
+      CHK(gemm(st, B, n, n, n, M(W), M(X2), one, one, sol[q], nn, n, nullptr));
+    }
+    // eps += alpha X_b + beta X_a ; eps_s += beta X_a ; alpha <- alpha X_a ; beta <- beta X_b
+    CHK(gemm(st, B, n, n, n, M(al), M(Bi), one, one, eps, nn, n, nullptr));
+    CHK(gemm(st, B, n, n, n, M(be), M(Ai), one, one, eps, nn, n, nullptr));
+    CHK(gemm(st, B, n, n, n, M(be), M(Ai), one, one, epss, nn, n, nullptr));
+    CHK(gemm(st, B, n, n, n, M(al), M(Ai), one, zero, Tm, nn, n, nullptr));
     std::swap(al, Tm);
-    CHK(gemm(st, B, n, n, n, M(Bi), M(be), one, zero, Tm, nn, n, nullptr));
+    CHK(gemm(st, B, n, n, n, M(be), M(Bi), one, zero, Tm, nn, n, nullptr));
     std::swap(be, Tm);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, st, remaining, 0, (int64_t)1);
     hipLaunchKernelGGL(ls_conv_kernel, dim3(B), dim3(256), 0, st, al, be, n, iters, done, remaining,

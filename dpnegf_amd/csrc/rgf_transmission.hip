@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void rgf_transmission_kernel(RgfArgs a) {
   const cx<double>* hd0 = base;
   const cx<double>* sd0 = base + n0 * n0;
   // FIXED: blocked inverse of g in place; otherwise the register-resident one
-  auto inverse_g = [&](int n) -> int {
+  auto inverse_g = [&](int n) __attribute__((always_inline)) -> int {
     if constexpr (FIXED)
       return wg_gj_inverse_blocked<R, 256, NMAX, LD>(
           g, [&](int i, int k, cx<R> v) { g[i * LD + k] = v; }, &gjb);

@@ -16,12 +16,14 @@ namespace negf {
 
 // C = sum_k A(i,k) B(k,j) for i<m, j<n, k<kd.  Every thread of the workgroup
 // must call it.  `epi(i, j, value)` is invoked once per output element after a
-// workgroup barrier that follows the last operand read.
-template <typename R, int NW, int MAXT, class FA, class FB, class EPI>
+// workgroup barrier that follows the last operand read.  SYNC = false drops
+// that barrier, for epilogues that write nothing any tile reads.  Tile t goes
+// to wave (t - ROT) mod NW, so ROT = 1 leaves wave 0 with the fewest tiles.
+template <typename R, int NW, int MAXT, bool SYNC = true, int ROT = 0, class FA, class FB, class EPI>
 __device__ __forceinline__ void wg_gemm(int m, int n, int kd, FA ga, FB gb, EPI epi) {
   typedef typename mfma<R>::v4 v4;
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = threadIdx.x / WAVE;
+  const int wave = (threadIdx.x / WAVE + NW - ROT) % NW;
   const int tn = (n + 15) >> 4;
   const int nt = ((m + 15) >> 4) * tn;
   v4 cr[MAXT], ci[MAXT];
@@ -49,7 +51,7 @@ __device__ __forceinline__ void wg_gemm(int m, int n, int kd, FA ga, FB gb, EPI 
       }
     }
   }
-  __syncthreads();
+  if (SYNC) __syncthreads();
 #pragma unroll
   for (int s = 0; s < MAXT; ++s) {
     const int t = wave + NW * s;
@@ -347,9 +349,9 @@ __device__ __forceinline__ int wg_gj_inverse_fixed(FL load, FS store, GJScratch<
 // ---- blocked Gauss-Jordan (matrix in LDS) -------------------------------------
 template <typename R, int N>
 struct GJBScratch {
-  cx<R> Z[4][N];  // pivot rows of the current panel, as they were before it
-  int pr[4];      // those rows
-  int piv[N];
+  cx<R> F[N][4];  // rank-4 multipliers of the current panel (inverse only)
+  int pr[4];      // pivot rows of the current panel
+  int piv[N];     // piv[j]: row pivoting column j
   int pinv[N];
   int singular;
 };
@@ -363,23 +365,42 @@ __device__ __forceinline__ double bcast_lane(double x, int src) {
 __device__ __forceinline__ float bcast_lane(float x, int src) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
 }
+template <typename R>
+__device__ __forceinline__ cx<R> bcast_lane(cx<R> x, int src) {
+  return {bcast_lane(x.re, src), bcast_lane(x.im, src)};
+}
 
-// In-place inverse of the N x N matrix A (LDS, leading dimension LDA), N <= 64
-// and a multiple of 4, workgroup of NT threads.  Panels of 4 columns are swept
-// by wave 0 (one row per lane; pivot = largest |re| + |im| among unused rows,
-// compared in single precision, lowest row on ties; the pivot row is broadcast
-// with readlane), then the rest of the matrix takes the panel's rank-4 update
-// on the matrix cores:  A[:, C] <- A[:, C] (pivot rows zeroed) + Jnew Z,  with
-// Jnew the swept panel and Z the pivot rows before the panel.  As in
-// wg_gj_inverse, rows are never swapped; store(i, k, v) receives the inverse
-// in natural order.  Returns nonzero if an exactly-zero pivot was met.
-template <typename R, int NT, int N, int LDA, class FS>
-__device__ __forceinline__ int wg_gj_inverse_blocked(cx<R>* A, FS store, GJBScratch<R, N>* sh) {
+// Blocked Gauss-Jordan on the N x NCOL matrix A in LDS (leading dimension LDA),
+// N <= 64 and a multiple of 4, workgroup of NT threads; rows are never
+// swapped (piv[j] is the row that pivots column j).
+//
+// Panels of BW = 4 columns.  Wave 0 sweeps a panel with exactly the
+// sequential algorithm (one row per lane; pivot = largest |re| + |im| among
+// unused rows, compared in single precision to 17 mantissa bits, lowest row on
+// ties, found with a DPP wave maximum; pivot row broadcast with readlane),
+// recording each row's multiplier F[i][k] (its column-k value at step k).  For
+// the other columns C the same steps are, exactly as in sequential order:
+//   pivot rows  u_k = (Z_k - sum_{k'<k} F[p_k][k'] u_k') / d_k   (Z: rows before
+//               the panel; forward substitution by wave 0, written over Z)
+//   all rows    A[i, C] -= sum_k Fm[i][k] u_k   (rank-4, matrix cores)
+// with Fm = F except that a pivot row p_k drops steps <= k (already in u_k).
+// This is the blocked-LU form of the update; applying the panel's combined
+// transform to the unreduced pivot rows instead (an inverse of the 4 x 4 pivot
+// block times Z) costs about two digits at ill-conditioned points.
+//
+// INV: in-place inverse of the N x N matrix (NCOL = N).  Panel columns keep
+//      the in-place inverse values, C = every other column, F lives in sh->F.
+// !INV (solve [W | B], NCOL = N + NC): C = columns right of the panel (swept
+//      columns of W are never needed again) and F is written over the dead
+//      panel columns.
+// Returns nonzero (uniformly) if an exactly-zero pivot was met.
+template <typename R, int NT, int N, int NCOL, int LDA, bool INV>
+__device__ __forceinline__ int wg_gj_blocked_core(cx<R>* A, GJBScratch<R, N>* sh) {
   constexpr int BW = 4;
   constexpr int NW = NT / WAVE;
-  constexpr int NTILE = ((N + 15) / 16) * ((N + 15) / 16);
+  constexpr int NTILE = ((N + 15) / 16) * ((NCOL - (INV ? 0 : BW) + 15) / 16);
   constexpr int MAXT = (NTILE + NW - 1) / NW;
-  static_assert(N <= WAVE && N % BW == 0, "blocked inverse: N <= 64, multiple of 4");
+  static_assert(N <= WAVE && N % BW == 0, "blocked Gauss-Jordan: N <= 64, multiple of 4");
   const int t = threadIdx.x;
   const int lane = t & (WAVE - 1);
   const int wave = t / WAVE;
@@ -388,74 +409,105 @@ __device__ __forceinline__ int wg_gj_inverse_blocked(cx<R>* A, FS store, GJBScra
   for (int j0 = 0; j0 < N; j0 += BW) {
     if (wave == 0) {
       const bool row_ok = lane < N;
-      cx<R> v[BW];
+      cx<R> v[BW], F[BW], dinv[BW], L[BW][BW];
+      int pr[BW];
+      int pstep = BW;  // step at which this lane's row pivots (BW: none)
 #pragma unroll
       for (int c = 0; c < BW; ++c) v[c] = row_ok ? A[lane * LDA + j0 + c] : cx<R>{0, 0};
 #pragma unroll
       for (int jj = 0; jj < BW; ++jj) {
-        unsigned long long key = 0;
+        unsigned key = 0;
         if (row_ok && !used) {
           const float kf = (float)pivot_key(v[jj]);
-          key = ((unsigned long long)(__float_as_uint(kf) + 1u) << 32) |
-                (unsigned long long)(unsigned int)(WAVE - 1 - lane);
+          key = (((__float_as_uint(kf) >> 6) + 1u) << 6) | (unsigned)(WAVE - 1 - lane);
         }
+        key = wave_max_u32(key);
+        int p = WAVE - 1 - (int)(key & 63u);
+        p = p < N ? p : N - 1;
+        pr[jj] = p;
+        const bool isp = (lane == p);
+        if (isp) { used = true; pstep = jj; }
+        F[jj] = v[jj];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-          const unsigned int lo = __shfl_xor((unsigned int)key, o, WAVE);
-          const unsigned int hi = __shfl_xor((unsigned int)(key >> 32), o, WAVE);
-          const unsigned long long ok = ((unsigned long long)hi << 32) | lo;
-          key = ok > key ? ok : key;
-        }
-        int p = WAVE - 1 - (int)(key & 0xffffffffull);
-        p = __builtin_amdgcn_readfirstlane(p < N ? p : N - 1);
-        if (lane == p) used = true;
+        for (int k2 = 0; k2 < jj; ++k2) L[jj][k2] = bcast_lane(F[k2], p);
         cx<R> prow[BW];
 #pragma unroll
-        for (int c = 0; c < BW; ++c) prow[c] = {bcast_lane(v[c].re, p), bcast_lane(v[c].im, p)};
+        for (int c = 0; c < BW; ++c) prow[c] = bcast_lane(v[c], p);
         const cx<R> d = prow[jj];
-        const cx<R> dinv = recip_fast(d);
+        dinv[jj] = recip_fast(d);
         if (lane == 0) {
           sh->piv[j0 + jj] = p;
           sh->pr[jj] = p;
           if (d.re == R(0) && d.im == R(0)) sh->singular = 1;
         }
-        const cx<R> fj = v[jj];
-        const bool isp = (lane == p);
+        const cx<R> fj = F[jj];
 #pragma unroll
         for (int c = 0; c < BW; ++c) {
           if (c == jj) {
-            v[c] = isp ? dinv : -(fj * dinv);
+            v[c] = isp ? dinv[jj] : -(fj * dinv[jj]);
           } else {
-            const cx<R> rc = prow[c] * dinv;
+            const cx<R> rc = prow[c] * dinv[jj];
             v[c] = isp ? rc : v[c] - fj * rc;
           }
         }
       }
+      cx<R> Fm[BW];
+#pragma unroll
+      for (int k = 0; k < BW; ++k) Fm[k] = (k <= pstep && pstep < BW) ? cx<R>{0, 0} : F[k];
       if (row_ok) {
 #pragma unroll
-        for (int c = 0; c < BW; ++c) A[lane * LDA + j0 + c] = v[c];
+        for (int c = 0; c < BW; ++c) {
+          if (INV) {
+            A[lane * LDA + j0 + c] = v[c];
+            sh->F[lane][c] = Fm[c];
+          } else {
+            A[lane * LDA + j0 + c] = Fm[c];
+          }
+        }
+      }
+      // pivot rows of the other columns: forward substitution in step order
+      const int clo = INV ? 0 : j0 + BW;
+      for (int c = clo + lane; c < NCOL; c += WAVE) {
+        if (INV && c >= j0 && c < j0 + BW) continue;
+        cx<R> u[BW];
+#pragma unroll
+        for (int k = 0; k < BW; ++k) {
+          cx<R> acc = A[pr[k] * LDA + c];
+#pragma unroll
+          for (int k2 = 0; k2 < k; ++k2) acc = acc - L[k][k2] * u[k2];
+          u[k] = acc * dinv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < BW; ++k) A[pr[k] * LDA + c] = u[k];
       }
     }
     __syncthreads();
-    for (int e = t; e < BW * N; e += NT) {
-      const int tt = e / N, c = e % N;
-      const bool inpanel = (c >= j0 && c < j0 + BW);
-      sh->Z[tt][c] = inpanel ? cx<R>{0, 0} : A[sh->pr[tt] * LDA + c];
-    }
-    __syncthreads();
-    for (int e = t; e < BW * N; e += NT) {
-      const int tt = e / N, c = e % N;
-      if (c < j0 || c >= j0 + BW) A[sh->pr[tt] * LDA + c] = cx<R>{0, 0};
-    }
-    __syncthreads();
-    wg_gemm<R, NW, MAXT>(
-        N, N, BW, [&](int i, int k) { return A[i * LDA + j0 + k]; },
-        [&](int k, int c) { return sh->Z[k][c]; },
+    const int c0 = INV ? 0 : j0 + BW;
+    const int p0 = sh->pr[0], p1 = sh->pr[1], p2 = sh->pr[2], p3 = sh->pr[3];
+    wg_gemm<R, NW, MAXT, true, 1>(
+        N, NCOL - c0, BW,
+        [&](int i, int k) { return INV ? sh->F[i][k] : A[i * LDA + j0 + k]; },
+        [&](int k, int c) {
+          const int pk = k == 0 ? p0 : k == 1 ? p1 : k == 2 ? p2 : p3;
+          return A[pk * LDA + c0 + c];
+        },
         [&](int i, int c, cx<R> val) {
-          if (c < j0 || c >= j0 + BW) A[i * LDA + c] = A[i * LDA + c] + val;
+          const int cc = c0 + c;
+          if (!INV || cc < j0 || cc >= j0 + BW) A[i * LDA + cc] = A[i * LDA + cc] - val;
         });
     __syncthreads();
   }
+  return sh->singular;
+}
+
+// In-place inverse of the N x N matrix A (LDS, leading dimension LDA) by the
+// blocked Gauss-Jordan above; store(i, k, v) receives the inverse in natural
+// order (A^-1[pinv[i]][piv[k]] = Y[i][k] for the swept Y).  Returns nonzero if
+// an exactly-zero pivot was met.
+template <typename R, int NT, int N, int LDA, class FS>
+__device__ __forceinline__ int wg_gj_inverse_blocked(cx<R>* A, FS store, GJBScratch<R, N>* sh) {
+  const int t = threadIdx.x;
+  const int sing = wg_gj_blocked_core<R, NT, N, N, LDA, true>(A, sh);
   for (int j = t; j < N; j += NT) sh->pinv[j] = j;
   __syncthreads();
   for (int j = t; j < N; j += NT) {
@@ -471,7 +523,6 @@ __device__ __forceinline__ int wg_gj_inverse_blocked(cx<R>* A, FS store, GJBScra
     val[s2] = (e < N * N) ? A[(e / N) * LDA + (e % N)] : cx<R>{0, 0};
   }
   __syncthreads();
-  const int sing = sh->singular;
 #pragma unroll
   for (int s2 = 0; s2 < PER; ++s2) {
     const int e = t + NT * s2;
@@ -479,6 +530,17 @@ __device__ __forceinline__ int wg_gj_inverse_blocked(cx<R>* A, FS store, GJBScra
   }
   __syncthreads();
   return sing;
+}
+
+// Solves W X = B by Gauss-Jordan elimination on the augmented A = [W | B]
+// (LDS, N x (N + NC), leading dimension LDA) with the blocked update above.
+// On return row piv[j] of the B columns holds row j of X:
+// X(j, c) = A[piv[j] * LDA + N + c].  Elimination on [W | B] is forward stable
+// like an LU solve with the stacked right-hand sides, which W^-1 B through an
+// explicit inverse is not.  Returns nonzero if an exactly-zero pivot was met.
+template <typename R, int NT, int N, int NC, int LDA>
+__device__ __forceinline__ int wg_gj_solve_blocked(cx<R>* A, GJBScratch<R, N>* sh) {
+  return wg_gj_blocked_core<R, NT, N, N + NC, LDA, false>(A, sh);
 }
 
 }  // namespace negf

@@ -119,8 +119,8 @@ def test_cpu_tensors_are_rejected():
 
 
 def _leads_sigmas_oracle(s, E, eta_lead):
-    """Oracle g_s and Sigma per energy, plus the spread between the two valid
-    orderings of the decimation (solve vs inverse): the agreement an
+    """Oracle g_s and Sigma per energy, plus the spread between two stable
+    orderings of the decimation (LU solve vs refined inverse): the agreement an
     ill-conditioned point (small eta near a band edge) allows at all."""
     out = []
     for lead in (s.lead_L, s.lead_R):
@@ -130,7 +130,7 @@ def _leads_sigmas_oracle(s, E, eta_lead):
             a = osg.lopez_sancho(hL.numpy(), hLL.numpy(), sL.numpy(), sLL.numpy(),
                                  complex(e) + 1j * eta_lead)
             b = osg.lopez_sancho(hL.numpy(), hLL.numpy(), sL.numpy(), sLL.numpy(),
-                                 complex(e) + 1j * eta_lead, variant="inverse")
+                                 complex(e) + 1j * eta_lead, variant="refined")
             it.append(a[1])
             gs.append(torch.tensor(a[0]))
             spread.append(np.abs(a[0] - b[0]).max() / np.abs(a[0]).max())
@@ -168,6 +168,27 @@ def test_lead_self_energy_matches_oracle(name):
     for lead, r in zip((s.lead_L, s.lead_R), ref):
         res = self_energy_batched([tuple(x.to(dev) for x in lead)], torch.tensor(E), eta,
                                   raise_on_error=False)
+        check_surface(res, r)
+
+
+# Points of the 4096-energy grid linspace(-3, 3, 4096) where the W=48 ribbon
+# lead's decimation is worst conditioned at eta = 1e-6 (Dyson residual of an
+# explicit-inverse decimation above 1e-3 there).
+EDGE_IDX = (137, 452, 1036, 1352, 1550, 2545, 2743, 3643)
+
+
+def test_lead_self_energy_band_edges_eta_1e6():
+    from dpnegf_amd.negf.surface_green import self_energy_batched
+    dev = _dev()
+    s = wl.ribbon(W=48, K=2, disorder=0.0)
+    E = np.linspace(-3, 3, 4096)[list(EDGE_IDX)]
+    print("case ribbon48 band edges", flush=True)
+    ref = _leads_sigmas_oracle(s, E, 1e-6)
+    for lead, r in zip((s.lead_L, s.lead_R), ref):
+        res = self_energy_batched([tuple(x.to(dev) for x in lead)], torch.tensor(E), 1e-6,
+                                  raise_on_error=False)
+        assert int((res.status & 4).sum().item()) == 0, res.resid
+        assert float(res.resid.max().item()) < 1e-6, res.resid
         check_surface(res, r)
 
 

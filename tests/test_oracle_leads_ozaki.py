@@ -117,6 +117,28 @@ def test_planted_wrong_lead_orientation_fails():
     assert abs(bad - _chain2_channels(0.3)) > 1e-2
 
 
+# Energies of the 4096-point grid linspace(-3, 3, 4096) where the W=48 ribbon lead's
+# decimation is ill-conditioned at eta = 1e-6 (grid indices 1550 and 1352).
+EDGE_E = [float(np.linspace(-3, 3, 4096)[i]) for i in (1550, 1352)]
+
+
+@pytest.mark.parametrize("E", EDGE_E)
+def test_lopez_sancho_association_near_band_edge(E):
+    """The reference solves W X = [alpha|beta] (surface_green.py:189-194).  That
+    association, and a refined inverse, keep the Dyson residual near 1e-7 here,
+    while W^-1 times [alpha|beta] fails the reference's own 3e-5 check: the
+    kernel must solve, not invert and multiply."""
+    s = wl.ribbon(W=48, K=2, disorder=0.0)
+    H, h01, _, S, s01, _ = [x.numpy() for x in s.lead_L]
+    z = E + 1e-6j
+    g_solve, it_s, r_solve = osg.lopez_sancho(H, h01, S, s01, z, variant="solve")
+    g_ref, it_r, r_ref = osg.lopez_sancho(H, h01, S, s01, z, variant="refined")
+    assert r_solve < 1e-6 and r_ref < 1e-6 and it_s == it_r == 29
+    assert np.abs(g_solve - g_ref).max() < 1e-4 * np.abs(g_solve).max()
+    with pytest.raises(ArithmeticError):
+        osg.lopez_sancho(H, h01, S, s01, z, variant="inverse")
+
+
 def test_ozaki_poles_and_fermi_identity():
     poles, res = ooz.ozaki_residues(M_cut=100)
     poles, res = poles.numpy(), res.numpy()
