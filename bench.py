@@ -26,6 +26,30 @@ sys.path.insert(0, ROOT)
 METRIC = "RGF (E,k) points/s + adjoint: mixed vs complex128 on 1 GPU at |ΔT|≤1e-6"
 PEAK_FP64 = 78.6e12
 PEAK_FP32 = 157.3e12
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rgf_transmission_kernel_ribbon48_K128_E4096_pmc.txt")
+COPY_BYTES = 1 << 30
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the transmission kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated); None if
+    that summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def ls_flops_per_point(n, iters):
+    """Algorithmic real flops of one Lopez-Sancho point, counted as the
+    reference's LU form: 50.7 n^3 per iteration (LU 8/3 n^3, solve with 2n
+    right-hand sides 16 n^3, four products 32 n^3), the final inverse 8 n^3 and
+    the residual check 24 n^3."""
+    return (50.7 * iters + 8.0 + 24.0) * n ** 3
 
 
 def rgf_flops_per_point(sizes, nL, nR):
@@ -54,13 +78,13 @@ def cpu_baseline(s, E, eta, n_sample):
     """Oracle (complex128 torch/scipy restatement of the reference) on a
     bounded, evenly spaced sample of the energy grid."""
     from oracle import device as odev, lead as olead, rgf as orgf, surface_green as osg
-    ncores = os.cpu_count() or 1
+    ncores = min(len(os.sched_getaffinity(0)), 16)
     torch.set_num_threads(ncores)
     idx = np.linspace(0, len(E) - 1, n_sample).round().astype(int)
     t0 = time.perf_counter()
     for i in idx:
         e = float(E[i])
-        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e), hDL=l[2], sDL=l[5],
+        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e, dtype=torch.float64), hDL=l[2], sDL=l[5],
                               etaLead=eta)[0] for l in (s.lead_L, s.lead_R)]
         ans = orgf.recursive_gf(torch.tensor(e, dtype=torch.complex128), s.hl, s.hd, s.hu, s.sd,
                                 s.su, s.sl, sig[0], sig[1], eta=eta, keep_gr_left=False)
@@ -80,6 +104,9 @@ def main():
     ap.add_argument("--energies", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed steps, copy 1 GiB device-to-device (a kernel of known "
+                         "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has a single MI355X and "
@@ -104,20 +131,26 @@ def main():
     T = torch.empty(nE, dtype=torch.float64, device=dev)
     status = torch.zeros(nE, dtype=torch.int32, device=dev)
 
+    lead_res = {}
+
     def step(ev=None):
-        sL = self_energy_batched(leads[0], E, eta, raise_on_error=False).sigma
-        sR = self_energy_batched(leads[1], E, eta, raise_on_error=False).sigma
+        if ev is not None:
+            ev[2].record()
+        rL = self_energy_batched(leads[0], E, eta, raise_on_error=False)
+        if ev is not None:
+            ev[3].record()
+        rR = self_energy_batched(leads[1], E, eta, raise_on_error=False)
         if ev is not None:
             ev[0].record()
-        rgf_transmission(bs, z, sL, sR, precision=args.precision, status=status, out=T)
+        rgf_transmission(bs, z, rL.sigma, rR.sigma, precision=args.precision, status=status, out=T)
         if ev is not None:
             ev[1].record()
-        return sL, sR
+        lead_res["L"], lead_res["R"] = rL, rR
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    events = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(4))
               for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -125,7 +158,37 @@ def main():
         step(events[k])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    rgf_ms = float(np.mean([a.elapsed_time(b) for a, b in events]))
+    rgf_ms = float(np.mean([ev[0].elapsed_time(ev[1]) for ev in events]))
+    ls_ms = float(np.mean([ev[2].elapsed_time(ev[3]) for ev in events]))
+    if args.copy_calibration:
+        src = torch.empty(COPY_BYTES // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    # Mixed path on the same inputs: the FP32 fused sweep, timed over 3 launches, and its
+    # largest |T - T_fp64| (the headline value stays on the FP64 path).
+    mixed = None
+    if args.precision == "fp64":
+        sL, sR = lead_res["L"].sigma, lead_res["R"].sigma
+        Tm = torch.empty_like(T)
+        rgf_transmission(bs, z, sL, sR, precision="mixed", out=Tm)
+        mev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        mev[0].record()
+        for _ in range(3):
+            rgf_transmission(bs, z, sL, sR, precision="mixed", out=Tm)
+        mev[1].record()
+        torch.cuda.synchronize()
+        mixed_ms = mev[0].elapsed_time(mev[1]) / 3
+        dT = (Tm - T).abs()
+        mixed = {"rgf_kernel_ms": round(mixed_ms, 4), "speedup_vs_fp64": round(rgf_ms / mixed_ms, 3),
+                 "max_abs_dT_vs_fp64": float(dT.max().item()),
+                 "points_over_1e-6": int((dT > 1e-6).sum().item()),
+                 "meets_abs_dT_1e-6": bool(dT.max().item() <= 1e-6)}
+    rL = lead_res["L"]
+    ls_iters = float(rL.iters.double().mean().item())
+    ls_flops = ls_flops_per_point(int(s.lead_L[0].shape[0]), ls_iters)
     ms_per_step = 1e3 * dt / args.steps
     points_per_s = nE / (dt / args.steps)
     flops = rgf_flops_per_point(bs.sizes, n0, bs.sizes[-1])
@@ -151,9 +214,18 @@ def main():
         "rgf_kernel_ms": round(rgf_ms, 4),
         "rgf_points_per_s": round(nE / (rgf_ms * 1e-3), 3),
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": profiled_traffic(),
                      "kernel": "rgf_transmission_kernel",
-                     "flops_per_point": flops},
+                     "flops_per_point": flops, "points_per_launch": nE},
+        "lead_kernel_ms": round(ls_ms, 4),
+        "roofline_lead": {"bound": "mfma", "kernel": "lead_self_energy_kernel",
+                          "achieved": ls_flops * nE / (ls_ms * 1e-3) / 1e12,
+                          "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                          "frac": ls_flops * nE / (ls_ms * 1e-3) / PEAK_FP64,
+                          "flops_per_point": ls_flops, "iterations_mean": ls_iters},
+        "lead_status_nonzero": int((rL.status != 0).sum().item()),
+        "lead_resid_max": float(rL.resid.max().item()),
+        "mixed_precision": mixed,
         "T_checksum": float(T.sum().item()),
         "status_nonzero": int((status != 0).sum().item()),
     }

@@ -23,7 +23,7 @@ def _dev():
 def _sigmas(s, E, eta):
     out = []
     for l in (s.lead_L, s.lead_R):
-        out.append(torch.stack([osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e), hDL=l[2],
+        out.append(torch.stack([osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e, dtype=torch.float64), hDL=l[2],
                                                sDL=l[5], etaLead=eta)[0] for e in E]))
     return out
 
@@ -63,7 +63,7 @@ def test_adjoint_matches_autograd_and_fd(case):
     g_hu = [h.to(dev).requires_grad_(True) for h in s.hu]
     g_hl = [h.to(dev).requires_grad_(True) for h in s.hl]
     sdv = lambda xs: [x.to(dev) for x in xs]
-    T = transmission(torch.tensor(E), g_hd, g_hu, g_hl, sdv(s.sd), sdv(s.su), sdv(s.sl),
+    T = transmission(torch.tensor(E, dtype=torch.float64), g_hd, g_hu, g_hl, sdv(s.sd), sdv(s.su), sdv(s.sl),
                      sL.to(dev), sR.to(dev), eta)
     Jg = (T * w.to(dev)).sum()
     assert abs(float(Jg) - float(J)) < 1e-10 * max(1, abs(float(J)))
@@ -87,7 +87,7 @@ def test_adjoint_matches_autograd_and_fd(case):
                 blocks = [x.detach().clone() for x in params]
                 blocks[p][i, j] += sgn * delta * direction
                 K = len(g_hd)
-                Tp = transmission(torch.tensor(E), blocks[:K], blocks[K:2 * K - 1],
+                Tp = transmission(torch.tensor(E, dtype=torch.float64), blocks[:K], blocks[K:2 * K - 1],
                                   blocks[2 * K - 1:], sdv(s.sd), sdv(s.su), sdv(s.sl), sL.to(dev),
                                   sR.to(dev), eta)
                 vals.append(float((Tp * w.to(dev)).sum()))

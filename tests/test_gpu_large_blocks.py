@@ -26,7 +26,7 @@ def test_batched_lopez_sancho_n64_matches_oracle():
     print("case ribbon64 leads", flush=True)
     ref = _leads_sigmas_oracle(s, E, eta)
     for lead, r in zip((s.lead_L, s.lead_R), ref):
-        res = self_energy_batched([tuple(x.to(dev) for x in lead)], torch.tensor(E), eta,
+        res = self_energy_batched([tuple(x.to(dev) for x in lead)], torch.tensor(E, dtype=torch.float64), eta,
                                   raise_on_error=False)
         check_surface(res, r)
 
@@ -40,11 +40,11 @@ def test_blocked_transmission_uneven_large_blocks():
     print("case multiorbital 72/90/81", flush=True)
     ref = []
     for e in E:
-        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e), hDL=l[2], sDL=l[5],
+        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e, dtype=torch.float64), hDL=l[2], sDL=l[5],
                               etaLead=eta)[0] for l in (s.lead_L, s.lead_R)]
         ans = orgf.recursive_gf(torch.tensor(e, dtype=torch.complex128), s.hl, s.hd, s.hu, s.sd,
                                 s.su, s.sl, sig[0], sig[1], eta=eta)
         ref.append(float(odev.transmission(ans[0], olead.sigma2gamma(sig[0]),
                                            olead.sigma2gamma(sig[1]))))
-    T = transmission_pipeline(s.to(dev), torch.tensor(E), eta_lead=eta, eta_device=eta).cpu().numpy()
+    T = transmission_pipeline(s.to(dev), torch.tensor(E, dtype=torch.float64), eta_lead=eta, eta_device=eta).cpu().numpy()
     assert np.abs(T - np.array(ref)).max() < 1e-10 * max(1.0, np.abs(ref).max()), (T, ref)

@@ -134,7 +134,7 @@ def _leads_sigmas_oracle(s, E, eta_lead):
             it.append(a[1])
             gs.append(torch.tensor(a[0]))
             spread.append(np.abs(a[0] - b[0]).max() / np.abs(a[0]).max())
-            sig.append(osg.selfEnergy(hL, hLL, sL, sLL, torch.tensor(e), hDL=hDL, sDL=sDL,
+            sig.append(osg.selfEnergy(hL, hLL, sL, sLL, torch.tensor(e, dtype=torch.float64), hDL=hDL, sDL=sDL,
                                       etaLead=eta_lead)[0])
         out.append((torch.stack(sig), torch.stack(gs), it, spread))
     return out
@@ -166,7 +166,7 @@ def test_lead_self_energy_matches_oracle(name):
     print(f"case {name}", flush=True)
     ref = _leads_sigmas_oracle(s, E, eta)
     for lead, r in zip((s.lead_L, s.lead_R), ref):
-        res = self_energy_batched([tuple(x.to(dev) for x in lead)], torch.tensor(E), eta,
+        res = self_energy_batched([tuple(x.to(dev) for x in lead)], torch.tensor(E, dtype=torch.float64), eta,
                                   raise_on_error=False)
         check_surface(res, r)
 
@@ -185,7 +185,7 @@ def test_lead_self_energy_band_edges_eta_1e6():
     print("case ribbon48 band edges", flush=True)
     ref = _leads_sigmas_oracle(s, E, 1e-6)
     for lead, r in zip((s.lead_L, s.lead_R), ref):
-        res = self_energy_batched([tuple(x.to(dev) for x in lead)], torch.tensor(E), 1e-6,
+        res = self_energy_batched([tuple(x.to(dev) for x in lead)], torch.tensor(E, dtype=torch.float64), 1e-6,
                                   raise_on_error=False)
         assert int((res.status & 4).sum().item()) == 0, res.resid
         assert float(res.resid.max().item()) < 1e-6, res.resid
@@ -195,7 +195,7 @@ def test_lead_self_energy_band_edges_eta_1e6():
 def _T_oracle(s, E, eta_lead, eta_dev):
     out = []
     for e in E:
-        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e), hDL=l[2], sDL=l[5],
+        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e, dtype=torch.float64), hDL=l[2], sDL=l[5],
                               etaLead=eta_lead)[0] for l in (s.lead_L, s.lead_R)]
         ans = orgf.recursive_gf(torch.tensor(e, dtype=torch.complex128), s.hl, s.hd, s.hu, s.sd,
                                 s.su, s.sl, sig[0], sig[1], eta=eta_dev)
@@ -218,6 +218,28 @@ def test_transmission_pipeline_matches_oracle(name):
         s, eta, E = wl.multiorbital(widths=(4, 6, 5), K=5, norb=3, seed=2), 1e-5, np.linspace(-2, 2, 7)
     print(f"case {name}", flush=True)
     ref = _T_oracle(s, E, eta, eta)
-    T = transmission_pipeline(s.to(dev), torch.tensor(E), eta_lead=eta, eta_device=eta).cpu().numpy()
+    T = transmission_pipeline(s.to(dev), torch.tensor(E, dtype=torch.float64), eta_lead=eta, eta_device=eta).cpu().numpy()
     err = np.abs(T - ref).max() / max(1.0, np.abs(ref).max())
     assert err < RTOL, (err, T, ref)
+
+
+def test_mixed_precision_transmission_close_to_fp64():
+    """FP32 fused sweep (precision="mixed") against the FP64 path on a
+    well-conditioned ribbon (W=16, K=16, eta=1e-3): FP32 rounding only, bounded
+    by 1e-3 absolute on T <= 16 (a wrong matrix-core layout gives O(1))."""
+    from dpnegf_amd.negf.recursive_green_cal import BTDSystems, rgf_transmission
+    from dpnegf_amd.negf.surface_green import self_energy_batched
+    dev = _dev()
+    s = wl.ribbon(W=16, K=16, seed=3).to(dev)
+    E = torch.linspace(-2.5, 2.5, 64, dtype=torch.float64, device=dev)
+    eta = 1e-3
+    print("case mixed ribbon16 K16", flush=True)
+    sL = self_energy_batched([s.lead_L], E, eta, raise_on_error=False).sigma
+    sR = self_energy_batched([s.lead_R], E, eta, raise_on_error=False).sigma
+    bs = BTDSystems([(s.hd, s.hu, s.hl, s.sd, s.su, s.sl)], dev)
+    z = (E.to(torch.complex128) + 1j * eta).contiguous()
+    T64 = rgf_transmission(bs, z, sL, sR, precision="fp64")
+    T32 = rgf_transmission(bs, z, sL, sR, precision="mixed")
+    err = float((T32 - T64).abs().max().item())
+    print(f"mixed max |dT| = {err:.3e}", flush=True)
+    assert err < 1e-3, err
