@@ -21,9 +21,12 @@
 
 namespace negf {
 
-// FIXED: every block has exactly NMAX rows (inverse with the compile-time layout)
+// FIXED: every block has exactly NMAX rows (compile-time shapes for the inverse
+// and the products).  Up to NMAX = 48 two workgroups share a CU (LDS), so the
+// register budget is held to two waves per SIMD.
 template <typename R, int NMAX, bool ORTHO, bool FIXED>
-__global__ __launch_bounds__(256) void rgf_transmission_kernel(RgfArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NMAX <= 48 ? 2 : 1)))
+void rgf_transmission_kernel(RgfArgs a) {
   constexpr int LD = NMAX + 1;
   constexpr int NT = (NMAX / 16) * (NMAX / 16);
   constexpr int MAXT_STACK = (2 * NT + 3) / 4;
@@ -104,35 +107,36 @@ __global__ __launch_bounds__(256) void rgf_transmission_kernel(RgfArgs a) {
     const bool last = (q + 1 == K - 1);
 
     // [P; g] <- [P; g] (U_q - z SU_q)
-    wg_gemm<R, 4, MAXT_STACK>(
-        n0 + nq, n1, nq, [&](int i, int k) { return PG[i * LD + k]; },
-        [&](int k, int j) {
-          cx<double> u = hu[k * n1 + j];
-          if (!ORTHO) u = u - zd * su[k * n1 + j];
-          return cvt<R>(u);
-        },
-        [&](int i, int j, cx<R> v) { PG[i * LD + j] = v; });
+    auto a1 = [&](int i, int k) { return PG[i * LD + k]; };
+    auto b1 = [&](int k, int j) {
+      cx<double> u = hu[k * n1 + j];
+      if (!ORTHO) u = u - zd * su[k * n1 + j];
+      return cvt<R>(u);
+    };
+    auto e1 = [&](int i, int j, cx<R> v) { PG[i * LD + j] = v; };
+    if constexpr (FIXED) wg_gemm_fixed<R, 4, 2 * NMAX, NMAX, NMAX, false, true>(a1, b1, e1);
+    else wg_gemm<R, 4, MAXT_STACK>(n0 + nq, n1, nq, a1, b1, e1);
     __syncthreads();
     // g <- -M_{q+1} - (L_q - z SL_q) Y
-    wg_gemm<R, 4, MAXT_SQ>(
-        n1, n1, nq,
-        [&](int i, int k) {
-          cx<double> l = hl[i * nq + k];
-          if (!ORTHO) l = l - zd * sl[i * nq + k];
-          return cvt<R>(l);
-        },
-        [&](int k, int j) { return g[k * LD + j]; },
-        [&](int i, int j, cx<R> v) {
-          g[i * LD + j] = cvt<R>(-(cvt<double>(v) + Mval(hd1, sd1, n1, i, j, false, last)));
-        });
+    auto a2 = [&](int i, int k) {
+      cx<double> l = hl[i * nq + k];
+      if (!ORTHO) l = l - zd * sl[i * nq + k];
+      return cvt<R>(l);
+    };
+    auto b2 = [&](int k, int j) { return g[k * LD + j]; };
+    auto e2 = [&](int i, int j, cx<R> v) {
+      g[i * LD + j] = cvt<R>(-(cvt<double>(v) + Mval(hd1, sd1, n1, i, j, false, last)));
+    };
+    if constexpr (FIXED) wg_gemm_fixed<R, 4, NMAX, NMAX, NMAX, true, false>(a2, b2, e2);
+    else wg_gemm<R, 4, MAXT_SQ>(n1, n1, nq, a2, b2, e2);
     __syncthreads();
-    sing |= inverse_g(n1);
-    __syncthreads();
+    sing |= inverse_g(n1);  // ends with a workgroup barrier
     // P <- P g_{q+1}
-    wg_gemm<R, 4, MAXT_SQ>(
-        n0, n1, n1, [&](int i, int k) { return P[i * LD + k]; },
-        [&](int k, int j) { return g[k * LD + j]; },
-        [&](int i, int j, cx<R> v) { P[i * LD + j] = v; });
+    auto a3 = [&](int i, int k) { return P[i * LD + k]; };
+    auto b3 = [&](int k, int j) { return g[k * LD + j]; };
+    auto e3 = [&](int i, int j, cx<R> v) { P[i * LD + j] = v; };
+    if constexpr (FIXED) wg_gemm_fixed<R, 4, NMAX, NMAX, NMAX, false, false>(a3, b3, e3);
+    else wg_gemm<R, 4, MAXT_SQ>(n0, n1, n1, a3, b3, e3);
     __syncthreads();
     off = off1;
   }

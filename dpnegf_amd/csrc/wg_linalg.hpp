@@ -67,6 +67,79 @@ __device__ __forceinline__ void wg_gemm(int m, int n, int kd, FA ga, FB gb, EPI 
   }
 }
 
+// wg_gemm for compile-time shapes (M, N multiples of 16, K a multiple of 4):
+// no bounds checks, and tiles past the end are clamped to a valid one (their
+// results are dropped), so the k-loop has no branches.  PFA / PFB: the A / B
+// operand of every tile for step k + 1 is loaded before the matrix-core
+// instructions of step k are issued, so a long-latency operand (a Hamiltonian
+// block read through L2) overlaps the MFMAs; LDS operands need no prefetch.
+// Same contract as wg_gemm otherwise.
+template <typename R, int NW, int M, int N, int K, bool PFA, bool PFB, bool SYNC = true,
+          int ROT = 0, class FA, class FB, class EPI>
+__device__ __forceinline__ void wg_gemm_fixed(FA ga, FB gb, EPI epi) {
+  typedef typename mfma<R>::v4 v4;
+  static_assert(M % 16 == 0 && N % 16 == 0 && K % 4 == 0, "wg_gemm_fixed shape");
+  constexpr int TN = N / 16;
+  constexpr int NT = (M / 16) * TN;
+  constexpr int MAXT = (NT + NW - 1) / NW;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = (threadIdx.x / WAVE + NW - ROT) % NW;
+  const int li = lane & 15, lk = lane >> 4;
+  int ti[MAXT], tj[MAXT];
+#pragma unroll
+  for (int s = 0; s < MAXT; ++s) {
+    const int t = w + NW * s;
+    const int tc = t < NT ? t : NT - 1;
+    ti[s] = (tc / TN) * 16 + li;
+    tj[s] = (tc % TN) * 16 + li;
+  }
+  v4 cr[MAXT], ci[MAXT];
+  cx<R> pa[PFA ? MAXT : 1], pb[PFB ? MAXT : 1];
+#pragma unroll
+  for (int s = 0; s < MAXT; ++s) {
+    cr[s] = v4{0, 0, 0, 0};
+    ci[s] = v4{0, 0, 0, 0};
+    if constexpr (PFA) pa[s] = ga(ti[s], lk);
+    if constexpr (PFB) pb[s] = gb(lk, tj[s]);
+  }
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    cx<R> na[PFA ? MAXT : 1], nb[PFB ? MAXT : 1];
+    if (k0 + 4 < K) {
+#pragma unroll
+      for (int s = 0; s < MAXT; ++s) {
+        if constexpr (PFA) na[s] = ga(ti[s], k0 + 4 + lk);
+        if constexpr (PFB) nb[s] = gb(k0 + 4 + lk, tj[s]);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < MAXT; ++s) {
+      cx<R> a, b;
+      if constexpr (PFA) a = pa[s]; else a = ga(ti[s], k0 + lk);
+      if constexpr (PFB) b = pb[s]; else b = gb(k0 + lk, tj[s]);
+      cr[s] = mfma<R>::op(a.re, b.re, cr[s]);
+      cr[s] = mfma<R>::op(-a.im, b.im, cr[s]);
+      ci[s] = mfma<R>::op(a.re, b.im, ci[s]);
+      ci[s] = mfma<R>::op(a.im, b.re, ci[s]);
+    }
+    if (k0 + 4 < K) {
+#pragma unroll
+      for (int s = 0; s < MAXT; ++s) {
+        if constexpr (PFA) pa[s] = na[s];
+        if constexpr (PFB) pb[s] = nb[s];
+      }
+    }
+  }
+  if (SYNC) __syncthreads();
+#pragma unroll
+  for (int s = 0; s < MAXT; ++s) {
+    if (w + NW * s < NT) {
+      const int ib = ti[s] - li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) epi(ib + mfma<R>::row(lk, r), tj[s], cx<R>{cr[s][r], ci[s][r]});
+    }
+  }
+}
+
 template <typename R, int NMAX>
 struct GJScratch {
   cx<R> col[2][NMAX];   // current pivot column, double-buffered across steps
