@@ -503,9 +503,12 @@ __device__ __forceinline__ int wg_gj_blocked_core(cx<R>* A, GJBScratch<R, N>* sh
         F[jj] = v[jj];
 #pragma unroll
         for (int k2 = 0; k2 < jj; ++k2) L[jj][k2] = bcast_lane(F[k2], p);
+        // The solve needs only the columns right of jj (the panel's own values
+        // are replaced by the multipliers); the inverse keeps all four.
         cx<R> prow[BW];
 #pragma unroll
-        for (int c = 0; c < BW; ++c) prow[c] = bcast_lane(v[c], p);
+        for (int c = 0; c < BW; ++c)
+          if (INV || c >= jj) prow[c] = bcast_lane(v[c], p);
         const cx<R> d = prow[jj];
         dinv[jj] = recip_fast(d);
         if (lane == 0) {
@@ -517,8 +520,8 @@ __device__ __forceinline__ int wg_gj_blocked_core(cx<R>* A, GJBScratch<R, N>* sh
 #pragma unroll
         for (int c = 0; c < BW; ++c) {
           if (c == jj) {
-            v[c] = isp ? dinv[jj] : -(fj * dinv[jj]);
-          } else {
+            if (INV) v[c] = isp ? dinv[jj] : -(fj * dinv[jj]);
+          } else if (INV || c > jj) {
             const cx<R> rc = prow[c] * dinv[jj];
             v[c] = isp ? rc : v[c] - fj * rc;
           }
