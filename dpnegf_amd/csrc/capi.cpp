@@ -151,7 +151,7 @@ int64_t dpnegf_rgf_full_workspace_elems(int K, const int32_t* h_bsz, int B) {
   int nmax = 0;
   if (!h_bsz || K < 1 || B < 0) return -1;
   for (int q = 0; q < K; ++q) nmax = h_bsz[q] > nmax ? h_bsz[q] : nmax;
-  return 5 * (int64_t)B * nmax * nmax;
+  return 5 * (int64_t)B * nmax * nmax + negf::inverse_workspace(nmax, B);
 }
 
 int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena, int B,
@@ -186,6 +186,7 @@ int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena
   a.s_out = (const cx<double>*)s_out; a.gip = (cx<double>*)gip; a.gpd = (cx<double>*)gpd;
   a.gpl = (cx<double>*)gpl; a.gpu = (cx<double>*)gpu;
   a.work = (cx<double>*)work; a.status = status;
+  a.inv_ws = nmax > negf::GJ_NMAX_LARGE ? a.work + 5 * (int64_t)B * nmax * nmax : nullptr;
   hipError_t e = negf::rgf_full(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "rgf_full");
   return 0;

@@ -100,6 +100,31 @@ __global__ void bconjt_kernel(cx<double>* D, const cx<double>* S, int m, int n, 
 }
 
 
+// D[b][i][k] = S[b][i][k] for a rows x cols sub-block (leading dimensions ldd / lds,
+// batch strides sd / ss)
+__global__ void bblock_copy_kernel(cx<double>* D, int ldd, int64_t sd, const cx<double>* S,
+                                   int lds, int64_t ss, int rows, int cols, int B) {
+  const int64_t tot = (int64_t)B * rows * cols;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < tot;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = x / ((int64_t)rows * cols);
+    const int e = (int)(x % ((int64_t)rows * cols));
+    const int i = e / cols, k = e % cols;
+    D[b * sd + (int64_t)i * ldd + k] = S[b * ss + (int64_t)i * lds + k];
+  }
+}
+
+// D[b] += I for B contiguous n x n matrices
+__global__ void badd_identity_kernel(cx<double>* D, int n, int B) {
+  const int64_t tot = (int64_t)B * n;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < tot;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = x / n;
+    const int i = (int)(x % n);
+    D[b * n * n + (int64_t)i * n + i].re += 1.0;
+  }
+}
+
 // out[i][k] = sum_b conj(in[b][k][i]);  in: [B, m, n], out: [n, m]
 __global__ void breduce_conjt_kernel(cx<double>* out, const cx<double>* in, int m, int n, int B) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -366,14 +391,84 @@ static hipError_t gemm(hipStream_t st, int B, int m, int n, int k, Opnd A, Opnd 
   return hipGetLastError();
 }
 
-static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* status) {
+#define CHK(x)                                  \
+  do {                                          \
+    hipError_t e_ = (x);                        \
+    if (e_ != hipSuccess) return e_;            \
+  } while (0)
+
+int64_t inverse_workspace(int n, int B) {
+  return n > GJ_NMAX_LARGE ? 3 * (int64_t)B * n * n : 0;
+}
+
+static hipError_t block_copy(hipStream_t st, cx<double>* D, int ldd, int64_t sd,
+                             const cx<double>* S, int lds, int64_t ss, int rows, int cols, int B) {
+  const int64_t tot = (int64_t)B * rows * cols;
+  hipLaunchKernelGGL(bblock_copy_kernel, dim3((int)std::min<int64_t>((tot + 255) / 256, 65535)),
+                     dim3(256), 0, st, D, ldd, sd, S, lds, ss, rows, cols, B);
+  return hipGetLastError();
+}
+
+// In-place inverse of B contiguous n x n matrices.  n <= GJ_NMAX_LARGE: one
+// workgroup per matrix, Gauss-Jordan with partial pivoting.  Larger n (up to
+// BLOCK_NMAX): two diagonal halves h1 = n/2, h2 = n - h1, each inverted with
+// partial pivoting, joined through the Schur complement S = A22 - A21 A11^-1 A12:
+//   A^-1 = [[X + X A12 Y A21 X, -X A12 Y], [-Y A21 X, Y]],  X = A11^-1, Y = S^-1.
+// No pivoting across the halves: every matrix inverted on this path is
+// z S - H - Sigma (or a decimation W = z S - eps) with eta > 0, whose
+// anti-Hermitian part is positive definite, so A11 and S are nonsingular --
+// but they can be far worse conditioned than A, so the assembled inverse takes
+// one Newton step in FP64, X <- X + X (I - A X), which squares its relative
+// error.  ws: inverse_workspace(n, B) elements.
+static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* status,
+                          cx<double>* ws) {
   const int64_t s = (int64_t)n * n;
   if (n <= 16) hipLaunchKernelGGL((binv_kernel<16, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
   else if (n <= 32) hipLaunchKernelGGL((binv_kernel<32, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
   else if (n <= 64) hipLaunchKernelGGL((binv_kernel<64, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
   else if (n <= 128) hipLaunchKernelGGL((binv_kernel<128, 1024>), dim3(B), dim3(1024), 0, st, A, n, s, status);
   else if (n <= GJ_NMAX_LARGE) hipLaunchKernelGGL((binv_kernel<GJ_NMAX_LARGE, 1024>), dim3(B), dim3(1024), 0, st, A, n, s, status);
-  else return hipErrorInvalidValue;
+  else if (n <= BLOCK_NMAX) {
+    if (!ws) return hipErrorInvalidValue;
+    const int h1 = n / 2, h2 = n - h1;
+    const int64_t s1 = (int64_t)h1 * h1, s2 = (int64_t)h2 * h2, s12 = (int64_t)h1 * h2;
+    cx<double>* A0 = ws;             // A as given             [n x n]
+    cx<double>* Xc = A0 + B * s;     // copy of the inverse    [n x n]
+    cx<double>* X = Xc + B * s;      // A11^-1   [h1 x h1]  (then I - A X, [n x n])
+    cx<double>* Y = X + B * s1;      // S^-1     [h2 x h2]
+    cx<double>* T1 = Y + B * s2;     // X A12    [h1 x h2]
+    cx<double>* T2 = T1 + B * s12;   // A21 X    [h2 x h1]
+    const cx<double> one{1, 0}, mone{-1, 0}, zero{0, 0};
+    cx<double>* A12 = A + h1;
+    cx<double>* A21 = A + (int64_t)h1 * n;
+    cx<double>* A22 = A21 + h1;
+    CHK(block_copy(st, A0, n, s, A, n, s, n, n, B));
+    CHK(block_copy(st, X, h1, s1, A, n, s, h1, h1, B));
+    CHK(inverse(st, X, h1, B, status, nullptr));
+    CHK(gemm(st, B, h1, h2, h1, mat(X, h1, s1, OP_N, nullptr, 0), mat(A12, n, s, OP_N, nullptr, 0), one, zero, T1, s12, h2, nullptr));
+    CHK(block_copy(st, Y, h2, s2, A22, n, s, h2, h2, B));
+    CHK(gemm(st, B, h2, h2, h1, mat(A21, n, s, OP_N, nullptr, 0), mat(T1, h2, s12, OP_N, nullptr, 0), mone, one, Y, s2, h2, nullptr));
+    CHK(inverse(st, Y, h2, B, status, nullptr));
+    CHK(gemm(st, B, h2, h1, h1, mat(A21, n, s, OP_N, nullptr, 0), mat(X, h1, s1, OP_N, nullptr, 0), one, zero, T2, s12, h1, nullptr));
+    CHK(gemm(st, B, h1, h2, h2, mat(T1, h2, s12, OP_N, nullptr, 0), mat(Y, h2, s2, OP_N, nullptr, 0), mone, zero, A12, s, n, nullptr));
+    CHK(gemm(st, B, h2, h1, h2, mat(Y, h2, s2, OP_N, nullptr, 0), mat(T2, h1, s12, OP_N, nullptr, 0), mone, zero, A21, s, n, nullptr));
+    CHK(block_copy(st, A, n, s, X, h1, s1, h1, h1, B));
+    CHK(gemm(st, B, h1, h1, h2, mat(A12, n, s, OP_N, nullptr, 0), mat(T2, h1, s12, OP_N, nullptr, 0), mone, one, A, s, n, nullptr));
+    CHK(block_copy(st, A22, n, s, Y, h2, s2, h2, h2, B));
+    // Newton step: R = I - A0 A^-1 (in X's space), A^-1 += copy(A^-1) R
+    cx<double>* Rm = X;
+    CHK(gemm(st, B, n, n, n, mat(A0, n, s, OP_N, nullptr, 0), mat(A, n, s, OP_N, nullptr, 0), mone,
+             zero, Rm, s, n, nullptr));
+    hipLaunchKernelGGL(badd_identity_kernel, dim3((int)std::min<int64_t>((B * (int64_t)n + 255) / 256, 65535)),
+                       dim3(256), 0, st, Rm, n, B);
+    CHK(hipGetLastError());
+    CHK(block_copy(st, Xc, n, s, A, n, s, n, n, B));
+    CHK(gemm(st, B, n, n, n, mat(Xc, n, s, OP_N, nullptr, 0), mat(Rm, n, s, OP_N, nullptr, 0), one,
+             one, A, s, n, nullptr));
+    return hipSuccess;
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -385,12 +480,6 @@ static Opnd mat(const cx<double>* h, int ld, int64_t bstride, int op, const cx<d
   o.h = h; o.s = s; o.bstride = bstride; o.sstride = sstride; o.ld = ld; o.op = op;
   return o;
 }
-
-#define CHK(x)                                  \
-  do {                                          \
-    hipError_t e_ = (x);                        \
-    if (e_ != hipSuccess) return e_;            \
-  } while (0)
 
 hipError_t rgf_full(const RgfFullArgs& a, hipStream_t st) {
   const int K = a.K, B = a.B;
@@ -437,7 +526,7 @@ hipError_t rgf_full(const RgfFullArgs& a, hipStream_t st) {
 
   // ---- forward: left-connected g_q  (:198-207)
   CHK(build(0, true, K == 1));
-  CHK(inverse(st, gl(0), n[0], B, a.status));
+  CHK(inverse(st, gl(0), n[0], B, a.status, a.inv_ws));
   for (int q = 0; q + 1 < K; ++q) {
     const int nq = n[q], n1 = n[q + 1];
     CHK(gemm(st, B, nq, n1, nq, mat(gl(q), nq, (int64_t)nq * nq), Uop(q), one, zero, w0,
@@ -445,7 +534,7 @@ hipError_t rgf_full(const RgfFullArgs& a, hipStream_t st) {
     CHK(build(q + 1, false, q + 1 == K - 1));
     CHK(gemm(st, B, n1, n1, nq, Lop(q), mat(w0, n1, (int64_t)nq * n1), mone, one, gl(q + 1),
              (int64_t)n1 * n1, n1, z));
-    CHK(inverse(st, gl(q + 1), n1, B, a.status));
+    CHK(inverse(st, gl(q + 1), n1, B, a.status, a.inv_ws));
   }
 
   // ---- backward: retarded blocks and the last column  (:209-238)
@@ -565,7 +654,7 @@ hipError_t rgf_full(const RgfFullArgs& a, hipStream_t st) {
 int64_t rgf_transmission_ops_workspace(int K, const int32_t* bsz, int B) {
   int nmax = 0;
   for (int q = 0; q < K; ++q) nmax = std::max(nmax, (int)bsz[q]);
-  return 8 * (int64_t)B * nmax * nmax;
+  return 8 * (int64_t)B * nmax * nmax + inverse_workspace(nmax, B);
 }
 
 // Transmission for any block size (<= GJ_NMAX_LARGE) by batched block operations:
@@ -592,6 +681,7 @@ hipError_t rgf_transmission_ops(const RgfOpsArgs& a, hipStream_t st) {
   cx<double>* gamR = gamL + m2;
   cx<double>* V = gamR + m2;
   cx<double>* W = V + m2;
+  cx<double>* iws = W + m2;  // inverse workspace
   const cx<double> one{1, 0}, mone{-1, 0}, zero{0, 0};
   const cx<double>* H = a.arena;
   auto hd = [&](int q) { return H + hoff[q]; };
@@ -611,7 +701,7 @@ hipError_t rgf_transmission_ops(const RgfOpsArgs& a, hipStream_t st) {
   };
   const int n0 = n[0];
   CHK(build(0, true, K == 1));
-  CHK(inverse(st, g, n0, B, a.status));
+  CHK(inverse(st, g, n0, B, a.status, iws));
   CHK(hipMemcpyAsync(P, g, sizeof(cx<double>) * B * n0 * n0, hipMemcpyDeviceToDevice, st));
   for (int q = 0; q + 1 < K; ++q) {
     const int nq = n[q], n1 = n[q + 1];
@@ -622,7 +712,7 @@ hipError_t rgf_transmission_ops(const RgfOpsArgs& a, hipStream_t st) {
     CHK(build(q + 1, false, q + 1 == K - 1));
     CHK(gemm(st, B, n1, n1, nq, Lop(q), mat(Y, n1, (int64_t)nq * n1), mone, one, g, (int64_t)n1 * n1,
              n1, a.z));
-    CHK(inverse(st, g, n1, B, a.status));
+    CHK(inverse(st, g, n1, B, a.status, iws));
     CHK(gemm(st, B, n0, n1, n1, mat(P2, n1, (int64_t)n0 * n1), mat(g, n1, (int64_t)n1 * n1), one,
              zero, P, (int64_t)n0 * n1, n1, a.z));
   }
@@ -645,7 +735,7 @@ hipError_t rgf_transmission_ops(const RgfOpsArgs& a, hipStream_t st) {
 
 int64_t lead_ls_batched_workspace(int n, int nred, int B) {
   const int m = std::max(n, nred);
-  return 9 * (int64_t)B * m * m + 4 * (int64_t)B;  // + int32 flags packed as complex slots
+  return 9 * (int64_t)B * m * m + inverse_workspace(m, B) + 4 * (int64_t)B;  // + int32 flags
 }
 
 hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) {
@@ -661,7 +751,8 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
   cx<double>* Tm = Bi + m2;
   cx<double>* X1 = Tm + m2;
   cx<double>* X2 = X1 + m2;
-  int32_t* flags = (int32_t*)(X2 + m2);  // done [B], iteration counts [B], remaining [1]
+  cx<double>* iws = X2 + m2;  // inverse workspace
+  int32_t* flags = (int32_t*)(iws + inverse_workspace(m, B));  // done [B], iteration counts [B], remaining [1]
   int32_t* done = flags;
   int32_t* iters = a.iters ? a.iters : flags + B;
   int32_t* remaining = flags + 2 * B;
@@ -690,7 +781,7 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
     // through the inverse, refined once in FP64, X += W^-1 (rhs - W X).  Without the
     // refinement, W^-1 times rhs loses the Dyson residual near band edges at small eta.
     CHK(hipMemcpyAsync(X1, W, sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
-    CHK(inverse(st, W, n, B, status));
+    CHK(inverse(st, W, n, B, status, iws));
     cx<double>* rhs[2] = {al, be};
     cx<double>* sol[2] = {Ai, Bi};
     for (int q = 0; q < 2; ++q) {
@@ -727,7 +818,7 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
   hipLaunchKernelGGL(ls_zs_minus_kernel, dim3(gridn), dim3(256), 0, st, W, epss, a.lead,
                      a.lead_stride, a.sys_idx, a.e, n, B, a.eta);
   CHK(hipGetLastError());
-  CHK(inverse(st, W, n, B, status));
+  CHK(inverse(st, W, n, B, status, iws));
   CHK(hipMemcpyAsync(a.gs, W, sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
   const cx<double>* gs = a.gs;
   // residual: X1 = (z s01 - h01) g_s ; X2 = X1 (z s10 - h10) ; R = (zS - H - X2) g_s
@@ -759,7 +850,7 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
              a.sigma, (int64_t)nr * nr, nr, nullptr));
   } else {
     CHK(hipMemcpyAsync(W, gs, sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
-    CHK(inverse(st, W, n, B, status));
+    CHK(inverse(st, W, n, B, status, iws));
     if (a.mode == 1) {
       // Sigma = (e S - H) - g_s^-1 : the coupling kernel with h = H (offset 0), s = S (2nn)
       hipLaunchKernelGGL(ls_coupling_kernel, dim3(gridn), dim3(256), 0, st, X1, a.lead,
@@ -784,7 +875,8 @@ int64_t rgf_grad_workspace(int K, const int32_t* bsz, int B) {
   }
   const int64_t m2 = (int64_t)nmax * nmax;
   // gl, grd | grl, gru | lc | g_trans | work(5) | gammaL, gammaR, C | R x2, G0 x2, D x2, W
-  return B * (2 * nsq + 2 * std::max<int64_t>(nx, 1) + nlc + m2 + 5 * m2 + 3 * m2 + 7 * m2);
+  return B * (2 * nsq + 2 * std::max<int64_t>(nx, 1) + nlc + m2 + 5 * m2 + 3 * m2 + 7 * m2) +
+         inverse_workspace(nmax, B);
 }
 
 hipError_t rgf_grad(const RgfGradArgs& g, hipStream_t st) {
@@ -819,6 +911,7 @@ hipError_t rgf_grad(const RgfGradArgs& g, hipStream_t st) {
   a.lc = w; w += (int64_t)B * nlc;
   a.g_trans = w; w += m2;
   a.work = w; w += 5 * m2;
+  a.inv_ws = g.work + (rgf_grad_workspace(K, g.bsz, B) - inverse_workspace(nmax, B));
   CHK(rgf_full(a, st));
   cx<double>* gamL = w; w += m2;
   cx<double>* gamR = w; w += m2;

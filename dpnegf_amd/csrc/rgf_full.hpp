@@ -5,7 +5,10 @@
 namespace negf {
 
 enum { OP_N = 0, OP_H = 1, OP_T = 2, OP_C = 3 };
-constexpr int GJ_NMAX_LARGE = 168;
+constexpr int GJ_NMAX_LARGE = 168;  // one-workgroup batched Gauss-Jordan inverse
+constexpr int BLOCK_NMAX = 256;     // two-level (Schur complement) inverse above that
+// Workspace of the batched inverse for n x n blocks (0 up to GJ_NMAX_LARGE).
+int64_t inverse_workspace(int n, int B);
 
 // Operand value (i, k) = op(h[b] - z_b s[b]) with s optional, row-major ld.
 struct Opnd {
@@ -45,6 +48,7 @@ struct RgfFullArgs {
   const cx<double>* s_out;     // greater source or null
   cx<double>* gip; cx<double>* gpd; cx<double>* gpl; cx<double>* gpu;
   cx<double>* work;            // 5 * B * nmax^2
+  cx<double>* inv_ws;          // inverse_workspace(nmax, B) (null when nmax <= GJ_NMAX_LARGE)
   int32_t* status;             // [B] or null
 };
 

@@ -48,3 +48,45 @@ def test_blocked_transmission_uneven_large_blocks():
                                            olead.sigma2gamma(sig[1]))))
     T = transmission_pipeline(s.to(dev), torch.tensor(E, dtype=torch.float64), eta_lead=eta, eta_device=eta).cpu().numpy()
     assert np.abs(T - np.array(ref)).max() < 1e-10 * max(1.0, np.abs(ref).max()), (T, ref)
+
+
+def test_recursive_gf_blocks_up_to_256():
+    """Blocks above 168 (two-level inverse): every output block of
+    recursive_gf for uneven blocks 256 / 192 / 240 with non-orthogonal S."""
+    from dpnegf_amd.negf.recursive_green_cal import recursive_gf
+    from tests.test_gpu_rgf import _cmp_tuple, _rand_se
+    dev = _dev()
+    sizes = [256, 192, 240]
+    print(f"case sizes={sizes}", flush=True)
+    hd, sd, hl, hu, sl, su = wl.random_btd(sizes, seed=7, overlap=True)
+    B = 2
+    sL, sR = _rand_se(B, sizes[0], 8), _rand_se(B, sizes[-1], 9)
+    E = torch.tensor([-0.6, 0.45], dtype=torch.float64) + 1e-3j
+    ref = orgf.recursive_gf(E, hl, hd, hu, sd, su, sl, sL, sR, eta=1e-4, need_gr_lc=True)
+    to = lambda xs: [x.to(dev) for x in xs]
+    out = recursive_gf(E.to(dev), to(hl), to(hd), to(hu), to(sd), to(su), to(sl), sL.to(dev),
+                       sR.to(dev), eta=1e-4, need_gr_lc=True)
+    _cmp_tuple(out, ref)
+
+
+def test_transmission_multiorbital_256_blocks():
+    """8 orbitals per site, layers of 32 / 24 / 30 sites (blocks 256 / 192 /
+    240): transmission through the blocked sweep with leads of 256 and 240
+    orbitals through the batched Lopez-Sancho path, against the oracle."""
+    from dpnegf_amd.negf.device_property import transmission_pipeline
+    dev = _dev()
+    s = wl.multiorbital(widths=(32, 24, 30), K=3, norb=8, seed=11)
+    E = np.array([-0.9, 0.25])
+    eta = 1e-4
+    print("case multiorbital 256/192/240", flush=True)
+    ref = []
+    for e in E:
+        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e, dtype=torch.float64),
+                              hDL=l[2], sDL=l[5], etaLead=eta)[0] for l in (s.lead_L, s.lead_R)]
+        ans = orgf.recursive_gf(torch.tensor(e, dtype=torch.complex128), s.hl, s.hd, s.hu, s.sd,
+                                s.su, s.sl, sig[0], sig[1], eta=eta)
+        ref.append(float(odev.transmission(ans[0], olead.sigma2gamma(sig[0]),
+                                           olead.sigma2gamma(sig[1]))))
+    T = transmission_pipeline(s.to(dev), torch.tensor(E, dtype=torch.float64), eta_lead=eta,
+                              eta_device=eta).cpu().numpy()
+    assert np.abs(T - np.array(ref)).max() < 1e-10 * max(1.0, np.abs(ref).max()), (T, ref)
