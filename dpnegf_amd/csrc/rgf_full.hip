@@ -426,7 +426,6 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
   if (n <= 16) hipLaunchKernelGGL((binv_kernel<16, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
   else if (n <= 32) hipLaunchKernelGGL((binv_kernel<32, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
   else if (n <= 64) hipLaunchKernelGGL((binv_kernel<64, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
-  else if (n <= 128) hipLaunchKernelGGL((binv_kernel<128, 1024>), dim3(B), dim3(1024), 0, st, A, n, s, status);
   else if (n <= GJ_NMAX_LARGE) hipLaunchKernelGGL((binv_kernel<GJ_NMAX_LARGE, 1024>), dim3(B), dim3(1024), 0, st, A, n, s, status);
   else if (n <= BLOCK_NMAX) {
     if (!ws) return hipErrorInvalidValue;
@@ -657,7 +656,7 @@ int64_t rgf_transmission_ops_workspace(int K, const int32_t* bsz, int B) {
   return 8 * (int64_t)B * nmax * nmax + inverse_workspace(nmax, B);
 }
 
-// Transmission for any block size (<= GJ_NMAX_LARGE) by batched block operations:
+// Transmission for any block size (<= BLOCK_NMAX) by batched block operations:
 // the same sweep as the fused kernel (P = G^L_{0,q} carried forward), then
 // T = Re Tr[Gamma_L Q Gamma_R Q^H] with Q the corner of P.
 hipError_t rgf_transmission_ops(const RgfOpsArgs& a, hipStream_t st) {

@@ -5,8 +5,11 @@
 namespace negf {
 
 enum { OP_N = 0, OP_H = 1, OP_T = 2, OP_C = 3 };
-constexpr int GJ_NMAX_LARGE = 168;  // one-workgroup batched Gauss-Jordan inverse
-constexpr int BLOCK_NMAX = 256;     // two-level (Schur complement) inverse above that
+// One-workgroup batched Gauss-Jordan inverse up to GJ_NMAX_LARGE (a 1024-thread
+// register-resident sweep; beyond 128 its column stripes spill), two-level
+// Schur-complement inverse with halves <= 128 above that, up to BLOCK_NMAX.
+constexpr int GJ_NMAX_LARGE = 128;
+constexpr int BLOCK_NMAX = 256;
 // Workspace of the batched inverse for n x n blocks (0 up to GJ_NMAX_LARGE).
 int64_t inverse_workspace(int n, int B);
 

@@ -50,13 +50,13 @@ def test_blocked_transmission_uneven_large_blocks():
     assert np.abs(T - np.array(ref)).max() < 1e-10 * max(1.0, np.abs(ref).max()), (T, ref)
 
 
-def test_recursive_gf_blocks_up_to_256():
-    """Blocks above 168 (two-level inverse): every output block of
-    recursive_gf for uneven blocks 256 / 192 / 240 with non-orthogonal S."""
+@pytest.mark.parametrize("sizes", [[256, 192, 240], [144, 168, 136]])
+def test_recursive_gf_blocks_up_to_256(sizes):
+    """Blocks above 128 (two-level inverse): every output block of
+    recursive_gf for uneven blocks with non-orthogonal S."""
     from dpnegf_amd.negf.recursive_green_cal import recursive_gf
     from tests.test_gpu_rgf import _cmp_tuple, _rand_se
     dev = _dev()
-    sizes = [256, 192, 240]
     print(f"case sizes={sizes}", flush=True)
     hd, sd, hl, hu, sl, su = wl.random_btd(sizes, seed=7, overlap=True)
     B = 2
