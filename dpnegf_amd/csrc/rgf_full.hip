@@ -21,38 +21,106 @@
 
 namespace negf {
 
-__device__ __forceinline__ cx<double> opnd_at(const Opnd& o, int b, int i, int k,
-                                              cx<double> zb) {
-  int r = i, c = k;
-  if (o.op == OP_H || o.op == OP_T) { r = k; c = i; }
-  const int64_t idx = (int64_t)b * o.bstride + (int64_t)r * o.ld + c;
-  cx<double> v = o.h[idx];
-  if (o.s) v = v - zb * o.s[(int64_t)b * o.sstride + (int64_t)r * o.ld + c];
-  if (o.op == OP_H || o.op == OP_C) v = conj(v);
-  return v;
+// An operand with its op resolved into strides: element (i, k) of op(h - z s)
+// sits at offset i * rs + k * cs of batch b, conjugated when csign < 0.
+struct OpView {
+  const cx<double>* h;
+  const cx<double>* s;
+  int64_t bstride, sstride, rs, cs;
+  double csign;
+};
+__device__ __forceinline__ OpView op_view(const Opnd& o) {
+  const bool tr = (o.op == OP_H || o.op == OP_T);
+  return {o.h, o.s, o.bstride, o.sstride, tr ? 1 : (int64_t)o.ld, tr ? (int64_t)o.ld : 1,
+          (o.op == OP_H || o.op == OP_C) ? -1.0 : 1.0};
+}
+template <bool HAS_S>
+__device__ __forceinline__ cx<double> view_at(const OpView& o, int b, int i, int k, cx<double> zb) {
+  const int64_t off = (int64_t)i * o.rs + (int64_t)k * o.cs;
+  cx<double> v = o.h[(int64_t)b * o.bstride + off];
+  if (HAS_S) v = v - zb * o.s[(int64_t)b * o.sstride + off];
+  return {v.re, o.csign * v.im};
 }
 
-// 64 x 64 output tile per workgroup, 4 waves x 4 tiles of 16 x 16.
+// 64 x 64 output tile per workgroup; wave w owns the 2 x 2 block of 16 x 16
+// tiles at rows 32 (w / 2), columns 32 (w % 2), so each k-step loads two A and
+// two B values per lane.  Indices past the edges are clamped (every load is in
+// bounds) and their values zeroed, so the k-loop has no branches, and the next
+// k-step's operands are loaded before this step's matrix-core instructions.
+template <bool SA, bool SB>
 __global__ __launch_bounds__(256) void bgemm_kernel(BGemmArgs a) {
+  typedef mfma<double>::v4 v4;
   const int tn = (a.n + 63) / 64;
   const int ti = blockIdx.x / tn, tj = blockIdx.x % tn;
   const int b = blockIdx.y;
   const int i0 = ti * 64, j0 = tj * 64;
-  const int mm = min(64, a.m - i0), nn = min(64, a.n - j0);
   const cx<double> zb = a.z ? a.z[b] : cx<double>{0, 0};
+  const OpView A = op_view(a.A), Bv = op_view(a.B_);
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const int li = lane & 15, lk = lane >> 4;
+  int ri[2], cj[2];
+  bool rok[2], cok[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int r = i0 + 32 * (w / 2) + 16 * t + li, c = j0 + 32 * (w % 2) + 16 * t + li;
+    rok[t] = r < a.m; cok[t] = c < a.n;
+    ri[t] = rok[t] ? r : a.m - 1;
+    cj[t] = cok[t] ? c : a.n - 1;
+  }
+  const int K = a.k;
+  auto ld_a = [&](int t, int k) -> cx<double> {
+    const bool ok = rok[t] && k < K;
+    const cx<double> v = view_at<SA>(A, b, ri[t], k < K ? k : K - 1, zb);
+    return {ok ? v.re : 0.0, ok ? v.im : 0.0};
+  };
+  auto ld_b = [&](int t, int k) -> cx<double> {
+    const bool ok = cok[t] && k < K;
+    const cx<double> v = view_at<SB>(Bv, b, k < K ? k : K - 1, cj[t], zb);
+    return {ok ? v.re : 0.0, ok ? v.im : 0.0};
+  };
+  v4 cr[2][2], ci[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) cr[x][y] = ci[x][y] = v4{0, 0, 0, 0};
+  cx<double> pa[2] = {ld_a(0, lk), ld_a(1, lk)};
+  cx<double> pb[2] = {ld_b(0, lk), ld_b(1, lk)};
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const int kn = k0 + 4 + lk;
+    const cx<double> na[2] = {ld_a(0, kn), ld_a(1, kn)};
+    const cx<double> nb[2] = {ld_b(0, kn), ld_b(1, kn)};
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        cr[x][y] = mfma<double>::op(pa[x].re, pb[y].re, cr[x][y]);
+        cr[x][y] = mfma<double>::op(-pa[x].im, pb[y].im, cr[x][y]);
+        ci[x][y] = mfma<double>::op(pa[x].re, pb[y].im, ci[x][y]);
+        ci[x][y] = mfma<double>::op(pa[x].im, pb[y].re, ci[x][y]);
+      }
+    pa[0] = na[0]; pa[1] = na[1];
+    pb[0] = nb[0]; pb[1] = nb[1];
+  }
   cx<double>* C = a.C + (int64_t)b * a.cstride;
-  const Opnd A = a.A, Bo = a.B_;
   const cx<double> alpha = a.alpha, beta = a.beta;
   const bool use_beta = (beta.re != 0.0 || beta.im != 0.0);
-  wg_gemm<double, 4, 4>(
-      mm, nn, a.k, [&](int i, int k) { return opnd_at(A, b, i0 + i, k, zb); },
-      [&](int k, int j) { return opnd_at(Bo, b, k, j0 + j, zb); },
-      [&](int i, int j, cx<double> v) {
-        cx<double>* c = C + (int64_t)(i0 + i) * a.ldc + (j0 + j);
-        cx<double> out = alpha * v;
-        if (use_beta) out = out + beta * (*c);
-        *c = out;
-      });
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      const int ib = i0 + 32 * (w / 2) + 16 * x;
+      const int j = j0 + 32 * (w % 2) + 16 * y + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = ib + mfma<double>::row(lk, r);
+        if (i < a.m && j < a.n) {
+          cx<double>* c = C + (int64_t)i * a.ldc + j;
+          cx<double> out = alpha * cx<double>{cr[x][y][r], ci[x][y][r]};
+          if (use_beta) out = out + beta * (*c);
+          *c = out;
+        }
+      }
+    }
 }
 
 template <int NMAX, int NT>
@@ -382,12 +450,17 @@ static hipError_t gemm(hipStream_t st, int B, int m, int n, int k, Opnd A, Opnd 
                        cx<double> alpha, cx<double> beta, cx<double>* C, int64_t cstride,
                        int ldc, const cx<double>* z) {
   if (m <= 0 || n <= 0) return hipSuccess;
+  if (k <= 0) return hipErrorInvalidValue;  // the kernel clamps indices into [0, k)
   BGemmArgs a;
   a.m = m; a.n = n; a.k = k; a.B = B;
   a.A = A; a.B_ = Bo; a.alpha = alpha; a.beta = beta;
   a.C = C; a.cstride = cstride; a.ldc = ldc; a.z = z;
   dim3 grid(((m + 63) / 64) * ((n + 63) / 64), B);
-  hipLaunchKernelGGL(bgemm_kernel, grid, dim3(256), 0, st, a);
+  const bool sa = A.s != nullptr, sb = Bo.s != nullptr;
+  if (sa && sb) hipLaunchKernelGGL((bgemm_kernel<true, true>), grid, dim3(256), 0, st, a);
+  else if (sa) hipLaunchKernelGGL((bgemm_kernel<true, false>), grid, dim3(256), 0, st, a);
+  else if (sb) hipLaunchKernelGGL((bgemm_kernel<false, true>), grid, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL((bgemm_kernel<false, false>), grid, dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
