@@ -33,17 +33,6 @@ template <typename R>
 __device__ __forceinline__ R abs1(cx<R> a) { return fabs(a.re) + fabs(a.im); }
 template <typename R>
 __device__ __forceinline__ R cabs(cx<R> a) { return sqrt(a.re * a.re + a.im * a.im); }
-// 1/d by Smith's scaling (avoids overflow of |d|^2 for large/small d)
-template <typename R>
-__device__ __forceinline__ cx<R> recip(cx<R> d) {
-  if (fabs(d.re) >= fabs(d.im)) {
-    R r = d.im / d.re, den = d.re + d.im * r;
-    return {R(1) / den, -r / den};
-  } else {
-    R r = d.re / d.im, den = d.re * r + d.im;
-    return {r / den, R(-1) / den};
-  }
-}
 template <typename R, typename S>
 __device__ __forceinline__ cx<R> cvt(cx<S> a) { return {R(a.re), R(a.im)}; }
 

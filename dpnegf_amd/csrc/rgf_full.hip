@@ -499,7 +499,8 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
   if (n <= 16) hipLaunchKernelGGL((binv_kernel<16, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
   else if (n <= 32) hipLaunchKernelGGL((binv_kernel<32, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
   else if (n <= 64) hipLaunchKernelGGL((binv_kernel<64, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
-  else if (n <= GJ_NMAX_LARGE) hipLaunchKernelGGL((binv_kernel<GJ_NMAX_LARGE, 1024>), dim3(B), dim3(1024), 0, st, A, n, s, status);
+  else if (n <= GJ_NMAX_LARGE)
+    hipLaunchKernelGGL((binv_kernel<GJ_NMAX_LARGE, 1024>), dim3(B), dim3(1024), 0, st, A, n, s, status);
   else if (n <= BLOCK_NMAX) {
     if (!ws) return hipErrorInvalidValue;
     const int h1 = n / 2, h2 = n - h1;
@@ -511,32 +512,33 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
     cx<double>* T1 = Y + B * s2;     // X A12    [h1 x h2]
     cx<double>* T2 = T1 + B * s12;   // A21 X    [h2 x h1]
     const cx<double> one{1, 0}, mone{-1, 0}, zero{0, 0};
+    auto M = [](const cx<double>* p, int ld, int64_t bs) {
+      return mat(p, ld, bs, OP_N, nullptr, 0);
+    };
     cx<double>* A12 = A + h1;
     cx<double>* A21 = A + (int64_t)h1 * n;
     cx<double>* A22 = A21 + h1;
     CHK(block_copy(st, A0, n, s, A, n, s, n, n, B));
     CHK(block_copy(st, X, h1, s1, A, n, s, h1, h1, B));
     CHK(inverse(st, X, h1, B, status, nullptr));
-    CHK(gemm(st, B, h1, h2, h1, mat(X, h1, s1, OP_N, nullptr, 0), mat(A12, n, s, OP_N, nullptr, 0), one, zero, T1, s12, h2, nullptr));
+    CHK(gemm(st, B, h1, h2, h1, M(X, h1, s1), M(A12, n, s), one, zero, T1, s12, h2, nullptr));
     CHK(block_copy(st, Y, h2, s2, A22, n, s, h2, h2, B));
-    CHK(gemm(st, B, h2, h2, h1, mat(A21, n, s, OP_N, nullptr, 0), mat(T1, h2, s12, OP_N, nullptr, 0), mone, one, Y, s2, h2, nullptr));
+    CHK(gemm(st, B, h2, h2, h1, M(A21, n, s), M(T1, h2, s12), mone, one, Y, s2, h2, nullptr));
     CHK(inverse(st, Y, h2, B, status, nullptr));
-    CHK(gemm(st, B, h2, h1, h1, mat(A21, n, s, OP_N, nullptr, 0), mat(X, h1, s1, OP_N, nullptr, 0), one, zero, T2, s12, h1, nullptr));
-    CHK(gemm(st, B, h1, h2, h2, mat(T1, h2, s12, OP_N, nullptr, 0), mat(Y, h2, s2, OP_N, nullptr, 0), mone, zero, A12, s, n, nullptr));
-    CHK(gemm(st, B, h2, h1, h2, mat(Y, h2, s2, OP_N, nullptr, 0), mat(T2, h1, s12, OP_N, nullptr, 0), mone, zero, A21, s, n, nullptr));
+    CHK(gemm(st, B, h2, h1, h1, M(A21, n, s), M(X, h1, s1), one, zero, T2, s12, h1, nullptr));
+    CHK(gemm(st, B, h1, h2, h2, M(T1, h2, s12), M(Y, h2, s2), mone, zero, A12, s, n, nullptr));
+    CHK(gemm(st, B, h2, h1, h2, M(Y, h2, s2), M(T2, h1, s12), mone, zero, A21, s, n, nullptr));
     CHK(block_copy(st, A, n, s, X, h1, s1, h1, h1, B));
-    CHK(gemm(st, B, h1, h1, h2, mat(A12, n, s, OP_N, nullptr, 0), mat(T2, h1, s12, OP_N, nullptr, 0), mone, one, A, s, n, nullptr));
+    CHK(gemm(st, B, h1, h1, h2, M(A12, n, s), M(T2, h1, s12), mone, one, A, s, n, nullptr));
     CHK(block_copy(st, A22, n, s, Y, h2, s2, h2, h2, B));
     // Newton step: R = I - A0 A^-1 (in X's space), A^-1 += copy(A^-1) R
     cx<double>* Rm = X;
-    CHK(gemm(st, B, n, n, n, mat(A0, n, s, OP_N, nullptr, 0), mat(A, n, s, OP_N, nullptr, 0), mone,
-             zero, Rm, s, n, nullptr));
-    hipLaunchKernelGGL(badd_identity_kernel, dim3((int)std::min<int64_t>((B * (int64_t)n + 255) / 256, 65535)),
-                       dim3(256), 0, st, Rm, n, B);
+    CHK(gemm(st, B, n, n, n, M(A0, n, s), M(A, n, s), mone, zero, Rm, s, n, nullptr));
+    const int gid = (int)std::min<int64_t>((B * (int64_t)n + 255) / 256, 65535);
+    hipLaunchKernelGGL(badd_identity_kernel, dim3(gid), dim3(256), 0, st, Rm, n, B);
     CHK(hipGetLastError());
     CHK(block_copy(st, Xc, n, s, A, n, s, n, n, B));
-    CHK(gemm(st, B, n, n, n, mat(Xc, n, s, OP_N, nullptr, 0), mat(Rm, n, s, OP_N, nullptr, 0), one,
-             one, A, s, n, nullptr));
+    CHK(gemm(st, B, n, n, n, M(Xc, n, s), M(Rm, n, s), one, one, A, s, n, nullptr));
     return hipSuccess;
   } else {
     return hipErrorInvalidValue;
@@ -824,7 +826,8 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
   cx<double>* X1 = Tm + m2;
   cx<double>* X2 = X1 + m2;
   cx<double>* iws = X2 + m2;  // inverse workspace
-  int32_t* flags = (int32_t*)(iws + inverse_workspace(m, B));  // done [B], iteration counts [B], remaining [1]
+  // flags: done [B], iteration counts [B], remaining [1]
+  int32_t* flags = (int32_t*)(iws + inverse_workspace(m, B));
   int32_t* done = flags;
   int32_t* iters = a.iters ? a.iters : flags + B;
   int32_t* remaining = flags + 2 * B;

@@ -311,114 +311,6 @@ __device__ __forceinline__ int wg_gj_inverse(int n, FL load, FS store, GJScratch
 
 
 
-// Row groups of the fixed-size inverse: a power of two with N / G rows each.
-constexpr int gj_groups(int N, int NT) {
-  int g = 1;
-  while (2 * g * N <= NT && N % (2 * g) == 0) g *= 2;
-  return g;
-}
-
-// Same algorithm and pivot rule as wg_gj_inverse for a matrix of exactly N
-// rows, with the thread layout fixed at compile time: G = gj_groups(N, NT)
-// row groups of N / G rows, so every register slot holds a valid row, the
-// pivot owner is p % G (a mask) and the partials reduce in registers.
-template <typename R, int NT, int N, class FL, class FS>
-__device__ __forceinline__ int wg_gj_inverse_fixed(FL load, FS store, GJScratch<R, N>* sh) {
-  constexpr int G = gj_groups(N, NT);
-  constexpr int MAXR = N / G;
-  static_assert(G * MAXR == N && G * N <= NT && MAXR <= 64, "layout");
-  const int t = threadIdx.x;
-  const int k = t % N, g = t / N;
-  const bool act = t < G * N;
-  uint64_t used = 0;
-  cx<R> v[MAXR];
-#pragma unroll
-  for (int r = 0; r < MAXR; ++r) v[r] = act ? load(g + G * r, k) : cx<R>{0, 0};
-  if (t == 0) sh->singular = 0;
-  if (act && k == 0) {
-    R best = R(-1);
-    int bi = N;
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) {
-      sh->col[0][g + G * r] = v[r];
-      const R m = pivot_key(v[r]);
-      if (m > best) { best = m; bi = g + G * r; }
-    }
-    sh->pval[0][g] = best;
-    sh->pidx[0][g] = bi;
-  }
-  __syncthreads();
-  for (int j = 0; j < N; ++j) {
-    const int b = j & 1;
-    R pv[G];
-    int pi[G];
-#pragma unroll
-    for (int q = 0; q < G; ++q) { pv[q] = sh->pval[b][q]; pi[q] = sh->pidx[b][q]; }
-    R best = pv[0];
-    int bi = pi[0];
-#pragma unroll
-    for (int q = 1; q < G; ++q)
-      if (pv[q] > best || (pv[q] == best && pi[q] < bi)) { best = pv[q]; bi = pi[q]; }
-    const int p = bi < N ? bi : N - 1;
-    const bool own = act && ((p & (G - 1)) == g);
-    if (own) used |= 1ull << (p / G);  // power-of-two G: a shift
-    if (act) {
-#pragma unroll
-      for (int r = 0; r < MAXR; ++r)
-        if (g + G * r == p) sh->row[k] = v[r];
-    }
-    __syncthreads();
-    const cx<R> d = sh->col[b][p];
-    if (t == 0) {
-      sh->piv[j] = p;
-      if (d.re == R(0) && d.im == R(0)) sh->singular = 1;
-    }
-    const cx<R> dinv = recip_fast(d);
-    const cx<R> rs = sh->row[k] * dinv;
-    const cx<R>* cb = sh->col[b];
-    const bool colj = (k == j);
-    // issue every column read before any arithmetic; updates are selects
-    cx<R> f[MAXR];
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) f[r] = cb[g + G * r];
-    const cx<R> prow = colj ? dinv : rs;
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) {
-      const cx<R> upd = colj ? -(f[r] * dinv) : v[r] - f[r] * rs;
-      v[r] = (g + G * r == p) ? prow : upd;
-    }
-    if (act && k == j + 1) {
-      R nbest = R(-1);
-      int nbi = N;
-#pragma unroll
-      for (int r = 0; r < MAXR; ++r) {
-        sh->col[b ^ 1][g + G * r] = v[r];
-        const R m = ((used >> r) & 1) ? R(-1) : pivot_key(v[r]);
-        if (m > nbest) { nbest = m; nbi = g + G * r; }
-      }
-      sh->pval[b ^ 1][g] = nbest;
-      sh->pidx[b ^ 1][g] = nbi;
-    }
-    __syncthreads();
-  }
-  for (int j = t; j < N; j += NT) sh->pinv[j] = j;
-  __syncthreads();
-  for (int j = t; j < N; j += NT) {
-    const int pj = sh->piv[j];
-    if (pj >= 0 && pj < N) sh->pinv[pj] = j;
-  }
-  __syncthreads();
-  const int sing = sh->singular;
-  if (act) {
-    const int pc = sh->piv[k];
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) store(sh->pinv[g + G * r], pc, v[r]);
-  }
-  __syncthreads();
-  return sing;
-}
-
-
 // ---- blocked Gauss-Jordan (matrix in LDS) -------------------------------------
 template <typename R, int N>
 struct GJBScratch {

@@ -49,13 +49,15 @@ class _Transmission(torch.autograd.Function):
         ghu = torch.empty(max(nx, 1), dtype=C128, device=dev)
         ghl = torch.empty(max(nx, 1), dtype=C128, device=dev)
         status = torch.zeros(B, dtype=torch.int32, device=dev)
-        nat.call("dpnegf_rgf_transmission_grad", nat.stream_ptr(dev), K, h_bsz, nat.ptr(bs.arena), B,
-                 nat.ptr(z), nat.ptr(sigL), sigL.shape[-1], nat.ptr(sigR), sigR.shape[-1],
-                 int(bs.ortho), nat.ptr(w), nat.ptr(ghd), nat.ptr(ghu), nat.ptr(ghl),
+        nat.call("dpnegf_rgf_transmission_grad", nat.stream_ptr(dev), K, h_bsz,
+                 nat.ptr(bs.arena), B, nat.ptr(z), nat.ptr(sigL), sigL.shape[-1],
+                 nat.ptr(sigR), sigR.shape[-1], int(bs.ortho), nat.ptr(w), nat.ptr(ghd),
+                 nat.ptr(ghu), nat.ptr(ghl),
                  nat.ptr(work), nat.ptr(status))
         g_hd = [t[0] for t in split_blocks(ghd, 1, [(n, n) for n in sizes])]
-        g_hu = [t[0] for t in split_blocks(ghu, 1, [(sizes[q], sizes[q + 1]) for q in range(K - 1)])]
-        g_hl = [t[0] for t in split_blocks(ghl, 1, [(sizes[q + 1], sizes[q]) for q in range(K - 1)])]
+        up = [(sizes[q], sizes[q + 1]) for q in range(K - 1)]
+        g_hu = [t[0] for t in split_blocks(ghu, 1, up)]
+        g_hl = [t[0] for t in split_blocks(ghl, 1, [(b, a) for a, b in up])]
         return (None, None, None, None, *g_hd, *g_hu, *g_hl)
 
 

@@ -115,9 +115,10 @@ def ozaki_density(blocks, lead_L, lead_R, kBT, mu_L, mu, R, poles, residues, E_r
         X = torch.empty(G, n, nK, dtype=C128, device=dev)
         Y = torch.empty(G, n, n, dtype=C128, device=dev)
         lcq = lc[q].contiguous()
-        nat.call("dpnegf_zgemm_batched", nat.stream_ptr(dev), G, n, nK, nK, nat.ptr(lcq), nK, n * nK,
-                 0, nat.ptr(gamR), nK, nK * nK, 0, 1.0, 0.0, 0.0, 0.0, nat.ptr(X), nK, n * nK)
-        nat.call("dpnegf_zgemm_batched", nat.stream_ptr(dev), G, n, n, nK, nat.ptr(X), nK, n * nK, 0,
+        st = nat.stream_ptr(dev)
+        nat.call("dpnegf_zgemm_batched", st, G, n, nK, nK, nat.ptr(lcq), nK, n * nK, 0,
+                 nat.ptr(gamR), nK, nK * nK, 0, 1.0, 0.0, 0.0, 0.0, nat.ptr(X), nK, n * nK)
+        nat.call("dpnegf_zgemm_batched", st, G, n, n, nK, nat.ptr(X), nK, n * nK, 0,
                  nat.ptr(lcq), nK, n * nK, 1, 1.0, 0.0, 0.0, 0.0, nat.ptr(Y), n, n * n)
         dm_neq.append(_weighted_blocks(Y.reshape(-1), G, [(n, n)], coef)[0])
     return dm_eq, dm_neq

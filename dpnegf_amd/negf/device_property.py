@@ -120,9 +120,11 @@ class DeviceProperty(object):
         idx1, idy1 = min(s11, se11), min(s12, se12)
         if block_tridiagonal:
             if se01 > s01 or se02 > s02:
-                raise ValueError("Left Lead Self Energy size is larger than the first Hamiltonian Block.")
+                raise ValueError("Left Lead Self Energy size is larger than the first "
+                                 "Hamiltonian Block.")
             if se11 > s11 or se12 > s12:
-                raise ValueError("Right Lead Self Energy size is larger than the last Hamiltonian Block.")
+                raise ValueError("Right Lead Self Energy size is larger than the last "
+                                 "Hamiltonian Block.")
         if need_lesser:
             if batched:
                 fL = self.lead_L.fermi_dirac(energy + self.E_ref).reshape(B, 1, 1)
@@ -130,8 +132,10 @@ class DeviceProperty(object):
                 seinL = 1j * (seL - seL.mH) * fL
                 seinR = 1j * (seR - seR.mH) * fR
             else:
-                seinL = (1j * (seL - seL.conj().T) * self.lead_L.fermi_dirac(energy + self.E_ref).reshape(-1)).unsqueeze(0)
-                seinR = (1j * (seR - seR.conj().T) * self.lead_R.fermi_dirac(energy + self.E_ref).reshape(-1)).unsqueeze(0)
+                fL = self.lead_L.fermi_dirac(energy + self.E_ref).reshape(-1)
+                fR = self.lead_R.fermi_dirac(energy + self.E_ref).reshape(-1)
+                seinL = (1j * (seL - seL.conj().T) * fL).unsqueeze(0)
+                seinR = (1j * (seR - seR.conj().T) * fR).unsqueeze(0)
             s_in = _build_s_in_batched(self.hd, seinL, seinR, idx0, idy0, idx1, idy1)
             if not batched:
                 s_in = [t[0] for t in s_in]

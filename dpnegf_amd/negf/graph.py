@@ -32,7 +32,8 @@ class TransmissionGraph:
         self.bs = BTDSystems(systems)
         dev = self.bs.device
         if max(self.bs.sizes) > nat.TRANSMISSION_NMAX:
-            raise NotImplementedError("graph capture needs block sizes <= %d" % nat.TRANSMISSION_NMAX)
+            raise NotImplementedError("graph capture needs block sizes <= %d"
+                                      % nat.TRANSMISSION_NMAX)
         if len(systems) != len(leads_L) or len(systems) != len(leads_R):
             raise ValueError("one left and one right lead per system")
         if len(systems) > 1 and sys_idx is None:
@@ -41,7 +42,8 @@ class TransmissionGraph:
         self.P = e.shape[0]
         self.e = e.contiguous()
         self.z = (self.e + 1j * float(eta_device)).contiguous()
-        self.sys_idx = None if sys_idx is None else sys_idx.to(device=dev, dtype=torch.int32).contiguous()
+        self.sys_idx = (None if sys_idx is None
+                        else sys_idx.to(device=dev, dtype=torch.int32).contiguous())
         self.eta_lead = float(eta_lead)
         self.leads = []
         for side in (leads_L, leads_R):

@@ -20,9 +20,10 @@ _BOLTZMANN = 1.380649e-23
 
 
 class LeadProperty(object):
-    def __init__(self, tab, hamiltonian, structure, results_path, voltage, structure_leads_fold=None,
-                 bloch_sorted_indice=None, useBloch=False, bloch_factor=[1, 1, 1], bloch_R_list=None,
-                 e_T=300, efermi=0.0, E_ref=None, device="cuda"):
+    def __init__(self, tab, hamiltonian, structure, results_path, voltage,
+                 structure_leads_fold=None, bloch_sorted_indice=None, useBloch=False,
+                 bloch_factor=[1, 1, 1], bloch_R_list=None, e_T=300, efermi=0.0, E_ref=None,
+                 device="cuda"):
         self.hamiltonian = hamiltonian
         self.structure = structure
         self.tab = tab
@@ -70,7 +71,6 @@ class LeadProperty(object):
                         se_numba_jit=None, HS_inmem=True):
         lead = self._lead_blocks(kpoint)
         e = torch.as_tensor(energy)
-        scalar = e.ndim == 0 or (e.ndim == 1 and e.numel() == 1 and torch.as_tensor(energy).ndim == 0)
         res = self_energy_batched([lead], e.reshape(-1), eta_lead, E_ref=self.E_ref)
         self.se = res.sigma[0] if e.ndim == 0 else res.sigma
         if not HS_inmem:

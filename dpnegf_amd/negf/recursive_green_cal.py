@@ -58,9 +58,11 @@ def recursive_gf(energy, hl, hd, hu, sd, su, sl, left_se, right_se, seP=None, E_
     B = e.shape[0]
     z = (e + 1j * eta).contiguous()
     if need_lesser:
-        assert isinstance(s_in, list), "need_lesser=True requires s_in to be a list of coupling matrices"
+        assert isinstance(s_in, list), \
+            "need_lesser=True requires s_in to be a list of coupling matrices"
     if need_greater:
-        assert isinstance(s_out, list), "need_greater=True requires s_out to be a list of coupling matrices"
+        assert isinstance(s_out, list), \
+            "need_greater=True requires s_out to be a list of coupling matrices"
 
     hd_eff = list(hd)
     if seP is not None:
@@ -131,7 +133,8 @@ def _run_full(z, arena, sizes, ortho, sigL, nL, sigR, nR, need_lc, s_in=None, s_
     out = {"gl": alloc(nsq), "grd": alloc(nsq), "grl": alloc(nx), "gru": alloc(nx),
            "lc": alloc(nlc) if need_lc else None,
            "g_trans": torch.empty(B, sizes[0], sizes[-1], dtype=C128, device=device)}
-    out["lesser"] = (s_in, alloc(nsq), alloc(nsq), alloc(nx), alloc(nx)) if s_in is not None else None
+    out["lesser"] = ((s_in, alloc(nsq), alloc(nsq), alloc(nx), alloc(nx))
+                     if s_in is not None else None)
     out["greater"] = ((s_out, alloc(nsq), alloc(nsq), alloc(nx), alloc(nx))
                       if s_out is not None else None)
     status = torch.zeros(B, dtype=torch.int32, device=device)

@@ -43,7 +43,8 @@ def pack_lead(hL, hLL, sL, sLL, hDL=None, sDL=None, device=None):
     n = hL.shape[-1]
     parts = [_dev(hL, device), _dev(hLL, device), _dev(sL, device), _dev(sLL, device)]
     if hDL is not None:
-        parts += [_dev(hDL, device), _dev(sDL if sDL is not None else torch.zeros_like(hDL), device)]
+        sDL = sDL if sDL is not None else torch.zeros_like(hDL)
+        parts += [_dev(hDL, device), _dev(sDL, device)]
     return torch.cat([p.contiguous().reshape(-1) for p in parts]), n
 
 

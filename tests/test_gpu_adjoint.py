@@ -11,6 +11,11 @@ import torch
 from oracle import device as odev, lead as olead, rgf as orgf, surface_green as osg
 from dpnegf_amd import workloads as wl
 
+
+def _f64(x):
+    """Energies as float64 (a Python float would give a float32 tensor)."""
+    return torch.tensor(x, dtype=torch.float64)
+
 pytestmark = pytest.mark.gpu
 
 
@@ -23,7 +28,7 @@ def _dev():
 def _sigmas(s, E, eta):
     out = []
     for l in (s.lead_L, s.lead_R):
-        out.append(torch.stack([osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e, dtype=torch.float64), hDL=l[2],
+        out.append(torch.stack([osg.selfEnergy(l[0], l[1], l[3], l[4], _f64(e), hDL=l[2],
                                                sDL=l[5], etaLead=eta)[0] for e in E]))
     return out
 
@@ -31,8 +36,9 @@ def _sigmas(s, E, eta):
 def _oracle_J(s, E, sL, sR, eta, w, hd, hu, hl):
     J = 0
     for b, e in enumerate(E):
-        ans = orgf.recursive_gf(torch.tensor(e, dtype=torch.complex128), hl, hd, hu, s.sd, s.su, s.sl,
-                                sL[b], sR[b], eta=eta, keep_gr_left=False)
+        z = torch.tensor(e, dtype=torch.complex128)
+        ans = orgf.recursive_gf(z, hl, hd, hu, s.sd, s.su, s.sl, sL[b], sR[b], eta=eta,
+                                keep_gr_left=False)
         J = J + w[b] * odev.transmission(ans[0], olead.sigma2gamma(sL[b]), olead.sigma2gamma(sR[b]))
     return J
 
@@ -63,7 +69,7 @@ def test_adjoint_matches_autograd_and_fd(case):
     g_hu = [h.to(dev).requires_grad_(True) for h in s.hu]
     g_hl = [h.to(dev).requires_grad_(True) for h in s.hl]
     sdv = lambda xs: [x.to(dev) for x in xs]
-    T = transmission(torch.tensor(E, dtype=torch.float64), g_hd, g_hu, g_hl, sdv(s.sd), sdv(s.su), sdv(s.sl),
+    T = transmission(_f64(E), g_hd, g_hu, g_hl, sdv(s.sd), sdv(s.su), sdv(s.sl),
                      sL.to(dev), sR.to(dev), eta)
     Jg = (T * w.to(dev)).sum()
     assert abs(float(Jg) - float(J)) < 1e-10 * max(1, abs(float(J)))
@@ -87,7 +93,7 @@ def test_adjoint_matches_autograd_and_fd(case):
                 blocks = [x.detach().clone() for x in params]
                 blocks[p][i, j] += sgn * delta * direction
                 K = len(g_hd)
-                Tp = transmission(torch.tensor(E, dtype=torch.float64), blocks[:K], blocks[K:2 * K - 1],
+                Tp = transmission(_f64(E), blocks[:K], blocks[K:2 * K - 1],
                                   blocks[2 * K - 1:], sdv(s.sd), sdv(s.su), sdv(s.sl), sL.to(dev),
                                   sR.to(dev), eta)
                 vals.append(float((Tp * w.to(dev)).sum()))

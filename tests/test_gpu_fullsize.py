@@ -90,7 +90,8 @@ def test_two_orbital_chain_201_energies():
                 H, h01, hDL, S, s01, sDL = [x.numpy() for x in l]
                 gs = osg.lopez_sancho(H, h01, S, s01, e + 1j * eta, variant=variant)[0]
                 b = hDL.shape[1]
-                sig.append(torch.tensor((e * sDL - hDL) @ gs[:b, :b] @ (e * sDL.conj().T - hDL.conj().T)))
+                tau = e * sDL - hDL
+                sig.append(torch.tensor(tau @ gs[:b, :b] @ tau.conj().T))
             ans = orgf.recursive_gf(torch.tensor(e, dtype=torch.complex128), s.hl, s.hd, s.hu, s.sd,
                                     s.su, s.sl, sig[0], sig[1], eta=eta, keep_gr_left=False)
             Ts.append(float(odev.transmission(ans[0], olead.sigma2gamma(sig[0]),

@@ -7,6 +7,11 @@ import torch
 from oracle import device as odev, lead as olead, rgf as orgf, surface_green as osg
 from dpnegf_amd import workloads as wl
 
+
+def _f64(x):
+    """Energies as float64 (a Python float would give a float32 tensor)."""
+    return torch.tensor(x, dtype=torch.float64)
+
 pytestmark = pytest.mark.gpu
 
 
@@ -26,7 +31,7 @@ def test_batched_lopez_sancho_n64_matches_oracle():
     print("case ribbon64 leads", flush=True)
     ref = _leads_sigmas_oracle(s, E, eta)
     for lead, r in zip((s.lead_L, s.lead_R), ref):
-        res = self_energy_batched([tuple(x.to(dev) for x in lead)], torch.tensor(E, dtype=torch.float64), eta,
+        res = self_energy_batched([tuple(x.to(dev) for x in lead)], _f64(E), eta,
                                   raise_on_error=False)
         check_surface(res, r)
 
@@ -40,13 +45,13 @@ def test_blocked_transmission_uneven_large_blocks():
     print("case multiorbital 72/90/81", flush=True)
     ref = []
     for e in E:
-        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e, dtype=torch.float64), hDL=l[2], sDL=l[5],
+        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], _f64(e), hDL=l[2], sDL=l[5],
                               etaLead=eta)[0] for l in (s.lead_L, s.lead_R)]
         ans = orgf.recursive_gf(torch.tensor(e, dtype=torch.complex128), s.hl, s.hd, s.hu, s.sd,
                                 s.su, s.sl, sig[0], sig[1], eta=eta)
         ref.append(float(odev.transmission(ans[0], olead.sigma2gamma(sig[0]),
                                            olead.sigma2gamma(sig[1]))))
-    T = transmission_pipeline(s.to(dev), torch.tensor(E, dtype=torch.float64), eta_lead=eta, eta_device=eta).cpu().numpy()
+    T = transmission_pipeline(s.to(dev), _f64(E), eta_lead=eta, eta_device=eta).cpu().numpy()
     assert np.abs(T - np.array(ref)).max() < 1e-10 * max(1.0, np.abs(ref).max()), (T, ref)
 
 
@@ -81,12 +86,12 @@ def test_transmission_multiorbital_256_blocks():
     print("case multiorbital 256/192/240", flush=True)
     ref = []
     for e in E:
-        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e, dtype=torch.float64),
+        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], _f64(e),
                               hDL=l[2], sDL=l[5], etaLead=eta)[0] for l in (s.lead_L, s.lead_R)]
         ans = orgf.recursive_gf(torch.tensor(e, dtype=torch.complex128), s.hl, s.hd, s.hu, s.sd,
                                 s.su, s.sl, sig[0], sig[1], eta=eta)
         ref.append(float(odev.transmission(ans[0], olead.sigma2gamma(sig[0]),
                                            olead.sigma2gamma(sig[1]))))
-    T = transmission_pipeline(s.to(dev), torch.tensor(E, dtype=torch.float64), eta_lead=eta,
+    T = transmission_pipeline(s.to(dev), _f64(E), eta_lead=eta,
                               eta_device=eta).cpu().numpy()
     assert np.abs(T - np.array(ref)).max() < 1e-10 * max(1.0, np.abs(ref).max()), (T, ref)

@@ -12,6 +12,11 @@ import torch
 from oracle import device as odev, lead as olead, rgf as orgf, surface_green as osg
 from dpnegf_amd import workloads as wl
 
+
+def _f64(x):
+    """Energies as float64 (a Python float would give a float32 tensor)."""
+    return torch.tensor(x, dtype=torch.float64)
+
 pytestmark = pytest.mark.gpu
 
 RTOL = 1e-10
@@ -134,7 +139,7 @@ def _leads_sigmas_oracle(s, E, eta_lead):
             it.append(a[1])
             gs.append(torch.tensor(a[0]))
             spread.append(np.abs(a[0] - b[0]).max() / np.abs(a[0]).max())
-            sig.append(osg.selfEnergy(hL, hLL, sL, sLL, torch.tensor(e, dtype=torch.float64), hDL=hDL, sDL=sDL,
+            sig.append(osg.selfEnergy(hL, hLL, sL, sLL, _f64(e), hDL=hDL, sDL=sDL,
                                       etaLead=eta_lead)[0])
         out.append((torch.stack(sig), torch.stack(gs), it, spread))
     return out
@@ -166,7 +171,7 @@ def test_lead_self_energy_matches_oracle(name):
     print(f"case {name}", flush=True)
     ref = _leads_sigmas_oracle(s, E, eta)
     for lead, r in zip((s.lead_L, s.lead_R), ref):
-        res = self_energy_batched([tuple(x.to(dev) for x in lead)], torch.tensor(E, dtype=torch.float64), eta,
+        res = self_energy_batched([tuple(x.to(dev) for x in lead)], _f64(E), eta,
                                   raise_on_error=False)
         check_surface(res, r)
 
@@ -185,7 +190,7 @@ def test_lead_self_energy_band_edges_eta_1e6():
     print("case ribbon48 band edges", flush=True)
     ref = _leads_sigmas_oracle(s, E, 1e-6)
     for lead, r in zip((s.lead_L, s.lead_R), ref):
-        res = self_energy_batched([tuple(x.to(dev) for x in lead)], torch.tensor(E, dtype=torch.float64), 1e-6,
+        res = self_energy_batched([tuple(x.to(dev) for x in lead)], _f64(E), 1e-6,
                                   raise_on_error=False)
         assert int((res.status & 4).sum().item()) == 0, res.resid
         assert float(res.resid.max().item()) < 1e-6, res.resid
@@ -195,7 +200,7 @@ def test_lead_self_energy_band_edges_eta_1e6():
 def _T_oracle(s, E, eta_lead, eta_dev):
     out = []
     for e in E:
-        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e, dtype=torch.float64), hDL=l[2], sDL=l[5],
+        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], _f64(e), hDL=l[2], sDL=l[5],
                               etaLead=eta_lead)[0] for l in (s.lead_L, s.lead_R)]
         ans = orgf.recursive_gf(torch.tensor(e, dtype=torch.complex128), s.hl, s.hd, s.hu, s.sd,
                                 s.su, s.sl, sig[0], sig[1], eta=eta_dev)
@@ -215,10 +220,11 @@ def test_transmission_pipeline_matches_oracle(name):
     elif name == "ribbon48":
         s, eta, E = wl.ribbon(W=48, K=6, seed=3), 1e-6, np.linspace(-3, 3, 5)
     else:
-        s, eta, E = wl.multiorbital(widths=(4, 6, 5), K=5, norb=3, seed=2), 1e-5, np.linspace(-2, 2, 7)
+        s = wl.multiorbital(widths=(4, 6, 5), K=5, norb=3, seed=2)
+        eta, E = 1e-5, np.linspace(-2, 2, 7)
     print(f"case {name}", flush=True)
     ref = _T_oracle(s, E, eta, eta)
-    T = transmission_pipeline(s.to(dev), torch.tensor(E, dtype=torch.float64), eta_lead=eta, eta_device=eta).cpu().numpy()
+    T = transmission_pipeline(s.to(dev), _f64(E), eta_lead=eta, eta_device=eta).cpu().numpy()
     err = np.abs(T - ref).max() / max(1.0, np.abs(ref).max())
     assert err < RTOL, (err, T, ref)
 
