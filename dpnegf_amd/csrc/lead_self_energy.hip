@@ -22,29 +22,66 @@
 // NMAX with an identity block in W and zero rows/columns in alpha and beta,
 // which leaves the n x n results exactly as without padding.
 //
-// LDS: AUG = [W | alpha | beta] (NMAX x 3 NMAX); after the solve W's columns
-// receive alpha again (from a copy in the workspace) and SP holds beta, so the
-// products read [alpha; beta] from [AUG[:, :N]; SP] and X from AUG through the
-// pivot order.  eps and the alpha copy live in the workspace, eps_s in the g_s
-// output; each is read and written once per iteration.
+// Layout: 3 NMAX/16 waves; wave w owns 16-column tile w of [W | alpha | beta]
+// in registers (the register-resident elimination of wg_gj_regs.hpp), so the
+// solve moves only panels through LDS.  After it the alpha / beta owners hold
+// X = [X_a | X_b] (rows in pivot order), which goes to LDS in matrix-core
+// B-operand fragment order; alpha and beta stay in LDS as A-operand fragments
+// from the previous iteration.  Then, with no data leaving the owners:
+//   alpha tile j owner:  alpha X_a (column tile j) -> its registers (next RHS)
+//   beta  tile j owner:  beta  X_b                 -> its registers
+//   W     tile j owner:  beta X_a -> eps_s += ;  + alpha X_b -> eps += ;
+//                        W = zS - eps for the next solve
+// eps lives in the workspace and eps_s in the g_s output, each read and
+// written by the same lanes every iteration.
 #include "rgf_kernels.hpp"
+#include "wg_gj_regs.hpp"
 
 namespace negf {
 
-template <int NMAX, bool ORTHO, int NTH>
-__global__ __launch_bounds__(NTH) void lead_self_energy_kernel(LeadArgs a) {
+template <int N>
+struct LsLoopShared {
+  static constexpr int NT16 = N / 16, KS = N / 4;
+  cx<double> Aa[NT16 * KS][WAVE];      // alpha, A-operand fragments (row tile, k-step)
+  cx<double> Ab[NT16 * KS][WAVE];      // beta
+  cx<double> XS[KS * 2 * NT16][WAVE];  // [X_a | X_b], B-operand fragments (k-step, col tile)
+  typename RegGJ<N>::Shared gj;
+};
+template <int N>
+struct LsPostShared {
+  cx<double> AUG[N * (3 * N + 1)];
+  cx<double> SP[N * (N + 1)];
+  GJBScratch<double, N> gjb;
+};
+template <int N>
+union LsShared {
+  LsLoopShared<N> loop;
+  LsPostShared<N> post;
+};
+
+template <int NMAX> constexpr int ls_threads() { return 3 * (NMAX / 16) * WAVE; }
+
+template <int NMAX, bool ORTHO>
+__global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(LeadArgs a) {
   typedef double R;
+  typedef RegGJ<NMAX> GJ;
+  typedef typename GJ::vE vE;
   constexpr int N = NMAX;
+  constexpr int NTH = ls_threads<NMAX>();
   constexpr int NW = NTH / WAVE;
+  constexpr int NT16 = N / 16;
+  constexpr int KS = N / 4;
+  constexpr int E = GJ::E;
   constexpr int LDG = 3 * N + 1;
   constexpr int LDP = N + 1;
   constexpr int NTILE = (N / 16) * (N / 16);
-  constexpr int MAXT2 = (2 * NTILE + NW - 1) / NW;
   constexpr int MAXT1 = (NTILE + NW - 1) / NW;
-  __shared__ cx<R> AUG[N * LDG];
-  __shared__ cx<R> SP[N * LDP];
-  __shared__ GJBScratch<R, N> gjb;
+  __shared__ LsShared<N> sh;
   __shared__ double red[NW];
+  cx<R>* AUG = sh.post.AUG;
+  cx<R>* SP = sh.post.SP;
+  GJBScratch<R, N>& gjb = sh.post.gjb;
+  auto& L = sh.loop;
 
   const int pt = blockIdx.x;
   if (pt >= a.P) return;
@@ -58,8 +95,7 @@ __global__ __launch_bounds__(NTH) void lead_self_energy_kernel(LeadArgs a) {
   const cx<R>* sDL = hDL + nr * n;
   const cx<R> e = a.e[pt];
   const cx<R> z = {e.re, e.im + a.eta};
-  cx<R>* eps = a.eps + (int64_t)pt * 2 * n * n;
-  cx<R>* acopy = eps + n * n;
+  cx<R>* eps = a.eps + (int64_t)pt * n * n;
   cx<R>* epss = a.gs + (int64_t)pt * n * n;
 
   auto zS = [&](int i, int k) -> cx<R> {
@@ -77,81 +113,203 @@ __global__ __launch_bounds__(NTH) void lead_self_energy_kernel(LeadArgs a) {
       AUG[i * LDG + k] = (i < n && k < n) ? f(i, k) : cx<R>{i == k ? R(1) : R(0), 0};
     }
   };
-  // alpha0 = h10 - z s10, beta0 = h01 - z s01, eps = eps_s = H   (:170-177)
-  for (int x = threadIdx.x; x < N * N; x += NTH) {
-    const int i = x / N, k = x % N;
-    cx<R> al = {0, 0}, be = {0, 0};
-    if (i < n && k < n) {
-      be = h01[i * n + k] - zs01(i, k);
-      // s10 = s01^H enters as z * conj(s01[k][i]); z itself is not conjugated
-      al = conj(h01[k * n + i]);
-      if (!ORTHO) al = al - z * conj(s01[k * n + i]);
-      acopy[i * n + k] = al;
-      eps[i * n + k] = H[i * n + k];
-      epss[i * n + k] = H[i * n + k];
-    }
-    AUG[i * LDG + N + k] = al;
-    AUG[i * LDG + 2 * N + k] = be;
-    SP[i * LDP + k] = be;
+
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  const int g = lane >> 4, c = lane & 15;
+  // Column tile of the wave: kind 0 W, 1 alpha, 2 beta.  With 9 waves dealt
+  // round-robin over the 4 SIMDs, waves 0, 4, 8 share one SIMD: they take the
+  // alpha tiles, and each W tile (twice the product work) shares a SIMD with
+  // one beta tile.
+  int kind = wave / NT16, jt = wave % NT16;
+  if (NT16 == 3) {
+    kind = (wave & 3) == 0 ? 1 : wave < 4 ? 0 : 2;
+    jt = kind == 1 ? wave >> 2 : kind == 0 ? wave - 1 : wave - 5;
   }
+  const int col = 16 * jt + c;
+  // A-operand fragment slot of element el (row 16 I + g + 4 r, column col):
+  // fragment I * KS + col / 4, lane g + 4 r + 16 (col % 4).
+  auto fslot = [&](int el) -> int {
+    const int I = el >> 2, r = el & 3;
+    return (I * KS + (col >> 2)) * WAVE + g + 4 * r + 16 * (col & 3);
+  };
+  cx<R>* const Aa = &L.Aa[0][0];
+  cx<R>* const Ab = &L.Ab[0][0];
+  vE xr, xi;
+  // alpha0 = h10 - z s10, beta0 = h01 - z s01, eps = eps_s = H, W = zS - H  (:170-177)
+#pragma unroll
+  for (int el = 0; el < E; ++el) {
+    const int row = GJ::row_of(el, g);
+    const bool in = row < n && col < n;
+    cx<R> v = {0, 0};
+    if (kind == 0) {
+      v = cx<R>{row == col ? R(1) : R(0), 0};
+      if (in) {
+        const cx<R> h = H[row * n + col];
+        eps[row * n + col] = h;
+        epss[row * n + col] = h;
+        v = zS(row, col) - h;
+      }
+    } else if (kind == 1) {
+      if (in) {
+        // s10 = s01^H enters as z * conj(s01[k][i]); z itself is not conjugated
+        v = conj(h01[col * n + row]);
+        if (!ORTHO) v = v - z * conj(s01[col * n + row]);
+      }
+      Aa[fslot(el)] = v;
+    } else {
+      if (in) v = h01[row * n + col] - zs01(row, col);
+      Ab[fslot(el)] = v;
+    }
+    xr[el] = v.re;
+    xi[el] = v.im;
+  }
+  if (threadIdx.x < N) L.gj.pinv[threadIdx.x] = threadIdx.x;
+  if (threadIdx.x == 0) L.gj.singular = 0;
+  __syncthreads();
 
   int status = 0;
   int it = 0;
   while (true) {
     ++it;
-    stage([&](int i, int k) { return zS(i, k) - eps[i * n + k]; });
-    __syncthreads();
-    status |= wg_gj_solve_blocked<R, NTH, N, 2 * N, LDG>(AUG, &gjb);
-    for (int x = threadIdx.x; x < N * N; x += NTH) {
-      const int i = x / N, k = x % N;
-      AUG[i * LDG + k] = (i < n && k < n) ? acopy[i * n + k] : cx<R>{0, 0};
+    // ---- [X_a | X_b] = W \ [alpha | beta] ----
+    bool used = false;
+    for (int p = 0; p < KS; ++p) {
+      const int j0 = 4 * p, buf = p & 1;
+      if (kind == 0 && jt == (j0 >> 4)) {
+        GJ::put_panel(xr, xi, j0 & 15, &L.gj);
+        GJ::sweep(j0, buf, used, &L.gj);
+      }
+      __syncthreads();
+      used = GJ::mark_used(used, buf, &L.gj);
+      // W tiles wholly left of the next panel are no longer needed
+      if (kind != 0 || 16 * jt + 15 >= j0 + 4) GJ::update(xr, xi, buf, &L.gj);
+    }
+    // X (register row i is row pinv[i] of X) to B-operand fragments
+    if (kind != 0) {
+      const int jc = (kind - 1) * NT16 + jt;
+#pragma unroll
+      for (int el = 0; el < E; ++el) {
+        const int k = L.gj.pinv[GJ::row_of(el, g)];
+        L.XS[(k >> 2) * 2 * NT16 + jc][16 * (k & 3) + c] = cx<R>{xr[el], xi[el]};
+      }
     }
     __syncthreads();
-    // [alpha; beta] X_a:  alpha X_a is the new alpha, beta X_a enters eps and eps_s
-    wg_gemm<R, NW, MAXT2>(
-        2 * N, N, N,
-        [&](int i, int k) { return i < N ? AUG[i * LDG + k] : SP[(i - N) * LDP + k]; },
-        [&](int k, int j) { return AUG[gjb.piv[k] * LDG + N + j]; },
-        [&](int i, int j, cx<R> v) {
-          if (i < N) {
-            AUG[i * LDG + N + j] = v;
-            if (i < n && j < n) acopy[i * n + j] = v;
-          } else {
-            const int r = i - N;
-            if (r < n && j < n) {
-              eps[r * n + j] = eps[r * n + j] + v;
-              epss[r * n + j] = epss[r * n + j] + v;
-            }
-          }
-        });
-    // [alpha; beta] X_b:  alpha X_b enters eps, beta X_b is the new beta.  The
-    // barrier inside orders these eps updates after the ones above.
-    wg_gemm<R, NW, MAXT2>(
-        2 * N, N, N,
-        [&](int i, int k) { return i < N ? AUG[i * LDG + k] : SP[(i - N) * LDP + k]; },
-        [&](int k, int j) { return AUG[gjb.piv[k] * LDG + 2 * N + j]; },
-        [&](int i, int j, cx<R> v) {
-          if (i < N) {
-            if (i < n && j < n) eps[i * n + j] = eps[i * n + j] + v;
-          } else {
-            const int r = i - N;
-            AUG[r * LDG + 2 * N + j] = v;
-            SP[r * LDP + j] = v;
-          }
-        });
+    // ---- products ----
+    // acc(I) += sum_kk A(I, kk) B(kk), operands of step kk + 1 loaded before
+    // the matrix-core steps of kk are issued
+    auto gemm_acc = [&](const cx<R>* A, const cx<R>* B) __attribute__((always_inline)) {
+      constexpr int BS = 2 * NT16 * WAVE;
+      cx<R> av[NT16], bv = B[lane];
+#pragma unroll
+      for (int I = 0; I < NT16; ++I) av[I] = A[I * KS * WAVE + lane];
+#pragma unroll
+      for (int kk = 0; kk < KS; ++kk) {
+        cx<R> na[NT16], nb;
+        if (kk + 1 < KS) {
+          nb = B[(kk + 1) * BS + lane];
+#pragma unroll
+          for (int I = 0; I < NT16; ++I) na[I] = A[(I * KS + kk + 1) * WAVE + lane];
+        }
+#pragma unroll
+        for (int I = 0; I < NT16; ++I) GJ::cmfma(xr, xi, I, av[I], bv);
+        if (kk + 1 < KS) {
+          bv = nb;
+#pragma unroll
+          for (int I = 0; I < NT16; ++I) av[I] = na[I];
+        }
+      }
+    };
+#pragma unroll
+    for (int el = 0; el < E; ++el) {
+      xr[el] = 0;
+      xi[el] = 0;
+    }
+    const cx<R>* XS = &L.XS[0][0];
+    if (kind == 0) {
+      // beta X_a enters eps_s and eps; alpha X_b then enters eps.  Row
+      // offsets and the constants of W = zS - eps come from values the
+      // compiler cannot hoist out of the loop: a dozen loop-invariant
+      // addresses and constants per matrix would otherwise be spilled and
+      // reloaded one by one before each access.  The read-modify-writes load
+      // every element before storing any (the compiler cannot prove the rows
+      // distinct, and interleaved accesses would each wait for memory).
+      int nq = n;
+      double one = 1.0, zr = z.re, zi = z.im;
+      asm volatile("" : "+s"(nq), "+v"(one), "+v"(zr), "+v"(zi));
+      const int lo = g * nq + col;
+      const bool cin = col < n;
+      cx<R> old[E];
+#pragma unroll
+      for (int el = 0; el < E; ++el) {
+        const int row = GJ::row_of(el, g);
+        const int off = lo + (row - g) * nq;
+        old[el] = (row < n && cin) ? epss[off] : cx<R>{0, 0};
+      }
+      gemm_acc(Ab, XS + jt * WAVE);
+#pragma unroll
+      for (int el = 0; el < E; ++el) {
+        const int row = GJ::row_of(el, g);
+        const int off = lo + (row - g) * nq;
+        if (row < n && cin) epss[off] = old[el] + cx<R>{xr[el], xi[el]};
+      }
+#pragma unroll
+      for (int el = 0; el < E; ++el) {
+        const int row = GJ::row_of(el, g);
+        const int off = lo + (row - g) * nq;
+        old[el] = (row < n && cin) ? eps[off] : cx<R>{0, 0};
+      }
+      gemm_acc(Aa, XS + (NT16 + jt) * WAVE);
+#pragma unroll
+      for (int el = 0; el < E; ++el) {
+        const int row = GJ::row_of(el, g);
+        const int off = lo + (row - g) * nq;
+        cx<R> w = {row == col ? one : R(0), 0};
+        if (row < n && cin) {
+          const cx<R> ne = old[el] + cx<R>{xr[el], xi[el]};
+          eps[off] = ne;
+          const cx<R> zq = {zr, zi};
+          const cx<R> zs = ORTHO ? (row == col ? zq : cx<R>{0, 0}) : zq * S[off];
+          w = zs - ne;
+        }
+        xr[el] = w.re;
+        xi[el] = w.im;
+      }
+    } else {
+      // alpha X_a (the new alpha) or beta X_b (the new beta)
+      gemm_acc(kind == 1 ? Aa : Ab, XS + ((kind - 1) * NT16 + jt) * WAVE);
+    }
     __syncthreads();
+    if (kind != 0) {
+      cx<R>* A = kind == 1 ? Aa : Ab;
+#pragma unroll
+      for (int el = 0; el < E; ++el) A[fslot(el)] = cx<R>{xr[el], xi[el]};
+    }
+    if (threadIdx.x < N) L.gj.pinv[threadIdx.x] = threadIdx.x;
+    __syncthreads();
+    // max_ij |alpha_ij| + |beta_ij|  (:205); the alpha owners read beta's fragments
     double m = 0.0;
-    for (int x = threadIdx.x; x < n * n; x += NTH) {
-      const int i = x / n, k = x % n;
-      m = fmax(m, cabs(AUG[i * LDG + N + k]) + cabs(AUG[i * LDG + 2 * N + k]));
+    if (kind == 1) {
+#pragma unroll
+      for (int el = 0; el < E; ++el) {
+        const int row = GJ::row_of(el, g);
+        if (row < n && col < n)
+          m = fmax(m, cabs(cx<R>{xr[el], xi[el]}) + cabs(Ab[fslot(el)]));
+      }
     }
-    m = block_max(m, red);
+    m = wave_max(m);
+    if (lane == 0) red[wave] = m;
+    __syncthreads();
+    m = red[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) m = fmax(m, red[w]);
     if (m < a.stop) break;
     if (it >= 101) {
       status |= STATUS_LS_NOCONV;
       break;
     }
   }
+  if (L.gj.singular) status |= STATUS_SINGULAR;
+  __syncthreads();
 
   // g_s = (zS - eps_s)^-1, written over eps_s (the g_s output)   (:209)
   stage([&](int i, int k) { return zS(i, k) - epss[i * n + k]; });
@@ -219,25 +377,17 @@ __global__ __launch_bounds__(NTH) void lead_self_energy_kernel(LeadArgs a) {
   }
 }
 
-#ifndef LS_THREADS_48
-#define LS_THREADS_48 512
-#endif
-// Threads per workgroup: n = 48 leaves one workgroup per CU (LDS), so it runs
-// 8 waves rather than 4 (measured faster with identical results).
-template <int NMAX> constexpr int ls_threads() { return NMAX == 48 ? LS_THREADS_48 : 256; }
-
 template <int NMAX>
 static hipError_t launch_ls(const LeadArgs& a, bool ortho, hipStream_t st) {
-  constexpr int NTH = ls_threads<NMAX>();
-  dim3 grid(a.P), block(NTH);
+  dim3 grid(a.P), block(ls_threads<NMAX>());
   if (ortho)
-    hipLaunchKernelGGL((lead_self_energy_kernel<NMAX, true, NTH>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((lead_self_energy_kernel<NMAX, true>), grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL((lead_self_energy_kernel<NMAX, false, NTH>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((lead_self_energy_kernel<NMAX, false>), grid, block, 0, st, a);
   return hipGetLastError();
 }
 
-int64_t lead_self_energy_workspace(int n, int P) { return 2 * (int64_t)P * n * n; }
+int64_t lead_self_energy_workspace(int n, int P) { return (int64_t)P * n * n; }
 
 hipError_t launch_lead_self_energy(const LeadArgs& a, bool ortho, hipStream_t st) {
   const int m = a.n > a.nred ? a.n : a.nred;

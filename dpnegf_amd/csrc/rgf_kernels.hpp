@@ -72,7 +72,7 @@ struct LeadArgs {
 };
 
 hipError_t launch_lead_self_energy(const LeadArgs& a, bool ortho, hipStream_t st);
-// Workspace (complex elements) of the fused kernel: eps and a copy of alpha per point.
+// Workspace (complex elements) of the fused kernel: eps per point.
 int64_t lead_self_energy_workspace(int n, int P);
 
 }  // namespace negf

@@ -1,0 +1,210 @@
+// Register-resident blocked Gauss-Jordan elimination (FP64, gfx950 wave64).
+//
+// The augmented matrix (N rows, N <= 64 a multiple of 16) is split into
+// 16-column tiles; each tile is owned by one wave and held in registers in the
+// accumulator layout of the 16x16x4 f64 matrix-core instruction: lane
+// (g = lane/16, c = lane%16) holds column c of the tile at rows
+// 16 I + g + 4 r in element e = 4 I + r of a vector of N/4 doubles (re and im
+// separately).  Only the current panel (4 columns), the rank-4 multipliers and
+// a few scalars pass through LDS, so a panel costs one workgroup barrier and
+// the trailing update runs entirely on register operands:
+//
+//   sweeper (owner of the panel's tile)  : copies the 4 panel columns to LDS,
+//       sweeps them with one row per lane (the sequential pivot steps), and
+//       publishes -Fm (matrix-core A operand), the pivot rows pr[k], the
+//       4 x 4 lower factor L and the pivot reciprocals; barrier;
+//   every owner, for its own columns     : gathers the 4 pivot-row values of
+//       each column (uniform register index + lane permute), forward-
+//       substitutes u_k = (z_k - sum_{k'<k} L[k][k'] u_k') / d_k, writes u_k
+//       over pivot row pr[k] and adds (-Fm) u with four matrix-core steps.
+//
+// This is the same arithmetic as the LDS-resident blocked form (pivot choice,
+// multipliers and the blocked-LU style update of the unreduced pivot rows);
+// only the storage differs.
+#pragma once
+#include "wg_linalg.hpp"
+
+namespace negf {
+
+// 1/d = conj(d) / |d|^2, with 1/|d|^2 from the hardware reciprocal refined by
+// two Newton steps (a few ulp; shorter dependency chain than the IEEE division
+// sequence).  Pivots are far from the overflow / underflow range of |d|^2.
+__device__ __forceinline__ cx<double> recip_nr(cx<double> d) {
+  const double m = d.re * d.re + d.im * d.im;
+  double r = __builtin_amdgcn_rcp(m);
+  double e = fma(-m, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-m, r, 1.0);
+  r = fma(r, e, r);
+  return {d.re * r, -d.im * r};
+}
+
+template <int N>
+struct RegGJ {
+  static_assert(N % 16 == 0 && N <= WAVE, "register Gauss-Jordan: N multiple of 16, <= 64");
+  static constexpr int NT16 = N / 16;  // row tiles
+  static constexpr int E = N / 4;      // elements per lane and tile column
+  typedef double vE __attribute__((ext_vector_type(E)));
+  typedef double v4 __attribute__((ext_vector_type(4)));
+
+  struct Shared {
+    cx<double> FM[2][NT16][WAVE];  // -Fm, A-operand fragment order, by panel parity
+    cx<double> PAN[N][4];          // the panel's columns, one row per entry
+    cx<double> L[2][4][4];
+    cx<double> dinv[2][4];
+    int pr[2][4];
+    int piv[N];                    // piv[j]: row that pivots column j
+    int pinv[N];                   // pinv[piv[j]] = j
+    int singular;
+  };
+
+  // Row held by element e of a lane in row group g.
+  static __device__ __forceinline__ int row_of(int e, int g) {
+    return 16 * (e >> 2) + g + 4 * (e & 3);
+  }
+
+  static __device__ __forceinline__ void cmfma(vE& xr, vE& xi, int I, cx<double> a,
+                                               cx<double> b) {
+    v4 cr = {xr[4 * I], xr[4 * I + 1], xr[4 * I + 2], xr[4 * I + 3]};
+    v4 ci = {xi[4 * I], xi[4 * I + 1], xi[4 * I + 2], xi[4 * I + 3]};
+    cr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, cr, 0, 0, 0);
+    cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, b.im, cr, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.im, ci, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, ci, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      xr[4 * I + r] = cr[r];
+      xi[4 * I + r] = ci[r];
+    }
+  }
+
+  // Sweeper, step 1: the panel's columns c0..c0+3 of the calling wave's tile
+  // to PAN (row-major), then a wave-local fence.
+  static __device__ __forceinline__ void put_panel(const vE& xr, const vE& xi, int c0,
+                                                   Shared* sh) {
+    const int lane = threadIdx.x & (WAVE - 1), g = lane >> 4, c = lane & 15;
+    if (c >= c0 && c < c0 + 4) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) sh->PAN[row_of(e, g)][c - c0] = cx<double>{xr[e], xi[e]};
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+
+  // Sweeper, step 2: the four pivot steps of panel j0 on PAN, one row per
+  // lane.  `used`: this lane's row has pivoted in an earlier panel.  Pivot =
+  // largest |re| + |im| among unused rows, compared in single precision to 17
+  // mantissa bits, lowest row on ties (a DPP wave maximum).  Publishes -Fm
+  // (Fm: the multipliers, zero for a pivot row's own and earlier steps), the
+  // pivot rows, L, the reciprocals, piv / pinv and the singular flag.
+  static __device__ __forceinline__ void sweep(int j0, int buf, bool used, Shared* sh) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const bool row_ok = lane < N;
+    cx<double> v[4], F[4], dinv[4], L[4][4];
+    int pr[4];
+    int pstep = 4;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = row_ok ? sh->PAN[lane][c] : cx<double>{0, 0};
+    bool sing = false;
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      unsigned key = 0;
+      if (row_ok && !used) {
+        const float kf = (float)pivot_key(v[jj]);
+        key = (((__float_as_uint(kf) >> 6) + 1u) << 6) | (unsigned)(WAVE - 1 - lane);
+      }
+      key = wave_max_u32(key);
+      int p = WAVE - 1 - (int)(key & 63u);
+      p = p < N ? p : N - 1;
+      pr[jj] = p;
+      const bool isp = (lane == p);
+      if (isp) { used = true; pstep = jj; }
+      F[jj] = v[jj];
+#pragma unroll
+      for (int k2 = 0; k2 < jj; ++k2) L[jj][k2] = bcast_lane(F[k2], p);
+      cx<double> prow[4];
+#pragma unroll
+      for (int c = jj; c < 4; ++c) prow[c] = bcast_lane(v[c], p);
+      const cx<double> d = prow[jj];
+      dinv[jj] = recip_nr(d);
+      sing |= (d.re == 0.0 && d.im == 0.0);
+      const cx<double> fj = F[jj];
+#pragma unroll
+      for (int c = jj + 1; c < 4; ++c) {
+        const cx<double> rc = prow[c] * dinv[jj];
+        v[c] = isp ? rc : v[c] - fj * rc;
+      }
+    }
+    if (row_ok) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const cx<double> fm = (k <= pstep && pstep < 4) ? cx<double>{0, 0} : F[k];
+        sh->FM[buf][lane >> 4][(lane & 15) + 16 * k] = -fm;
+      }
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        sh->pr[buf][k] = pr[k];
+        sh->dinv[buf][k] = dinv[k];
+        sh->piv[j0 + k] = pr[k];
+        sh->pinv[pr[k]] = j0 + k;
+#pragma unroll
+        for (int k2 = 0; k2 < k; ++k2) sh->L[buf][k][k2] = L[k][k2];
+      }
+      if (sing) sh->singular = 1;
+    }
+  }
+
+  // After the panel's barrier, every wave: marks the panel's pivot rows as
+  // used (lane = row).
+  static __device__ __forceinline__ bool mark_used(bool used, int buf, const Shared* sh) {
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) used |= (lane == sh->pr[buf][k]);
+    return used;
+  }
+
+  // After the panel's barrier, an owner: elimination step on its tile.
+  static __device__ __forceinline__ void update(vE& xr, vE& xi, int buf, const Shared* sh) {
+    const int lane = threadIdx.x & (WAVE - 1), g = lane >> 4, c = lane & 15;
+    int pe[4], pg[4];
+    cx<double> z[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int p = __builtin_amdgcn_readfirstlane(sh->pr[buf][k]);
+      pe[k] = 4 * (p >> 4) + ((p & 15) >> 2);
+      pg[k] = p & 3;
+      const double zr = xr[pe[k]], zi = xi[pe[k]];
+      const int src = 16 * pg[k] + c;
+      z[k] = cx<double>{__shfl(zr, src, WAVE), __shfl(zi, src, WAVE)};
+    }
+    cx<double> u[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      cx<double> acc = z[k];
+#pragma unroll
+      for (int k2 = 0; k2 < k; ++k2) acc = acc - sh->L[buf][k][k2] * u[k2];
+      u[k] = acc * sh->dinv[buf][k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool own = (g == pg[k]);
+      const double cr = xr[pe[k]], ci = xi[pe[k]];
+      xr[pe[k]] = own ? u[k].re : cr;
+      xi[pe[k]] = own ? u[k].im : ci;
+    }
+    // B operand: lane group g supplies step g.  Scalar selects: a conditional
+    // between two array elements selects their addresses and leaves u[] in
+    // scratch memory.
+    const bool g1 = g & 1, g2 = g & 2;
+    const double b01r = g1 ? u[1].re : u[0].re, b01i = g1 ? u[1].im : u[0].im;
+    const double b23r = g1 ? u[3].re : u[2].re, b23i = g1 ? u[3].im : u[2].im;
+    const cx<double> b = {g2 ? b23r : b01r, g2 ? b23i : b01i};
+#pragma unroll
+    for (int I = 0; I < NT16; ++I) cmfma(xr, xi, I, sh->FM[buf][I][lane], b);
+  }
+};
+
+}  // namespace negf
