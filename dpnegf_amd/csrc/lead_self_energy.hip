@@ -39,13 +39,15 @@
 
 namespace negf {
 
+// The small, hot structures come first: LDS instructions address up to 64 KB
+// through their immediate offset, beyond that each address needs a register.
 template <int N>
 struct LsLoopShared {
   static constexpr int NT16 = N / 16, KS = N / 4;
+  typename RegGJ<N>::Shared gj;
   cx<double> Aa[NT16 * KS][WAVE];      // alpha, A-operand fragments (row tile, k-step)
   cx<double> Ab[NT16 * KS][WAVE];      // beta
   cx<double> XS[KS * 2 * NT16][WAVE];  // [X_a | X_b], B-operand fragments (k-step, col tile)
-  typename RegGJ<N>::Shared gj;
 };
 template <int N>
 struct LsPostShared {
@@ -54,9 +56,12 @@ struct LsPostShared {
   GJBScratch<double, N> gjb;
 };
 template <int N>
-union LsShared {
-  LsLoopShared<N> loop;
-  LsPostShared<N> post;
+struct LsShared {
+  double red[16];
+  union {
+    LsLoopShared<N> loop;
+    LsPostShared<N> post;
+  };
 };
 
 template <int NMAX> constexpr int ls_threads() { return 3 * (NMAX / 16) * WAVE; }
@@ -76,8 +81,9 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
   constexpr int LDP = N + 1;
   constexpr int NTILE = (N / 16) * (N / 16);
   constexpr int MAXT1 = (NTILE + NW - 1) / NW;
+  static_assert(NW <= 16, "reduction slots");
   __shared__ LsShared<N> sh;
-  __shared__ double red[NW];
+  double* const red = sh.red;
   cx<R>* AUG = sh.post.AUG;
   cx<R>* SP = sh.post.SP;
   GJBScratch<R, N>& gjb = sh.post.gjb;
@@ -127,10 +133,12 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
   }
   const int col = 16 * jt + c;
   // A-operand fragment slot of element el (row 16 I + g + 4 r, column col):
-  // fragment I * KS + col / 4, lane g + 4 r + 16 (col % 4).
+  // fragment I * KS + col / 4, lane g + 4 r + 16 (col % 4), stored at that
+  // lane XOR 4 ((col / 4) % 4) so that the owners' scattered writes spread
+  // over all LDS banks (a reader of k-step kk applies the same XOR).
   auto fslot = [&](int el) -> int {
     const int I = el >> 2, r = el & 3;
-    return (I * KS + (col >> 2)) * WAVE + g + 4 * r + 16 * (col & 3);
+    return (I * KS + (col >> 2)) * WAVE + ((g + 4 * r + 16 * (col & 3)) ^ (4 * ((col >> 2) & 3)));
   };
   cx<R>* const Aa = &L.Aa[0][0];
   cx<R>* const Ab = &L.Ab[0][0];
@@ -173,16 +181,62 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
     ++it;
     // ---- [X_a | X_b] = W \ [alpha | beta] ----
     bool used = false;
-    for (int p = 0; p < KS; ++p) {
-      const int j0 = 4 * p, buf = p & 1;
-      if (kind == 0 && jt == (j0 >> 4)) {
-        GJ::put_panel(xr, xi, j0 & 15, &L.gj);
-        GJ::sweep(j0, buf, used, &L.gj);
+    if constexpr (NT16 == 3) {
+      // The sweeping wave of W tile ts copies its tile to one row per lane
+      // (v = the first panel, xr / xi = the other twelve columns) and sweeps
+      // its four panels there, applying each pivot step to its remaining
+      // columns itself.  It needs no matrix-core update of its own tile, and
+      // after publishing a panel it starts on the next one while the other
+      // waves apply the published one.
+      // Panel q of W tile ts (q a compile-time constant: the row length).
+      auto panel = [&](auto qc, int ts, bool swp, cx<R>(&v)[4]) __attribute__((always_inline)) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int NC = 16 - 4 * q;  // panel + further columns of the tile
+        const int j0 = 16 * ts + 4 * q, buf = (j0 >> 2) & 1;
+        if (swp) {
+          cx<R> row[NC];
+#pragma unroll
+          for (int m = 0; m < NC; ++m) {
+            const int src = m + 4 * q - 4;  // register pair of column 4 q + m (v for the first panel)
+            if (src < 0) row[m] = v[m < 4 ? m : 0];
+            else row[m] = cx<R>{xr[src], xi[src]};
+          }
+          GJ::template sweep_row<NC>(row, j0, buf, used, &L.gj);
+#pragma unroll
+          for (int m = 0; m < NC; ++m) {
+            const int dst = m + 4 * q - 4;
+            if (dst >= 0) {
+              xr[dst] = row[m].re;
+              xi[dst] = row[m].im;
+            }
+          }
+        }
+        __syncthreads();
+        used = GJ::mark_used(used, buf, &L.gj);
+        // W tiles left of the next panel, and the sweeper's own, need no update
+        if (!swp && (kind != 0 || jt > ts)) GJ::update(xr, xi, buf, &L.gj);
+      };
+      for (int ts = 0; ts < NT16; ++ts) {
+        const bool swp = kind == 0 && jt == ts;
+        cx<R> v[4];
+        if (swp) GJ::tile_to_rows(xr, xi, v, &L.XS[0][0]);
+        panel(std::integral_constant<int, 0>{}, ts, swp, v);
+        panel(std::integral_constant<int, 1>{}, ts, swp, v);
+        panel(std::integral_constant<int, 2>{}, ts, swp, v);
+        panel(std::integral_constant<int, 3>{}, ts, swp, v);
       }
-      __syncthreads();
-      used = GJ::mark_used(used, buf, &L.gj);
-      // W tiles wholly left of the next panel are no longer needed
-      if (kind != 0 || 16 * jt + 15 >= j0 + 4) GJ::update(xr, xi, buf, &L.gj);
+    } else {
+      for (int p = 0; p < KS; ++p) {
+        const int j0 = 4 * p, buf = p & 1;
+        if (kind == 0 && jt == (j0 >> 4)) {
+          GJ::put_panel(xr, xi, j0 & 15, &L.gj);
+          GJ::sweep(j0, buf, used, &L.gj);
+        }
+        __syncthreads();
+        used = GJ::mark_used(used, buf, &L.gj);
+        // W tiles wholly left of the next panel are no longer needed
+        if (kind != 0 || 16 * jt + 15 >= j0 + 4) GJ::update(xr, xi, buf, &L.gj);
+      }
     }
     // X (register row i is row pinv[i] of X) to B-operand fragments
     if (kind != 0) {
@@ -201,14 +255,15 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
       constexpr int BS = 2 * NT16 * WAVE;
       cx<R> av[NT16], bv = B[lane];
 #pragma unroll
-      for (int I = 0; I < NT16; ++I) av[I] = A[I * KS * WAVE + lane];
+      for (int I = 0; I < NT16; ++I) av[I] = A[I * KS * WAVE + lane];  // k-step 0: no XOR
 #pragma unroll
       for (int kk = 0; kk < KS; ++kk) {
         cx<R> na[NT16], nb;
         if (kk + 1 < KS) {
           nb = B[(kk + 1) * BS + lane];
 #pragma unroll
-          for (int I = 0; I < NT16; ++I) na[I] = A[(I * KS + kk + 1) * WAVE + lane];
+          for (int I = 0; I < NT16; ++I)
+            na[I] = A[(I * KS + kk + 1) * WAVE + (lane ^ (4 * ((kk + 1) & 3)))];
         }
 #pragma unroll
         for (int I = 0; I < NT16; ++I) GJ::cmfma(xr, xi, I, av[I], bv);
@@ -238,6 +293,7 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
       asm volatile("" : "+s"(nq), "+v"(one), "+v"(zr), "+v"(zi));
       const int lo = g * nq + col;
       const bool cin = col < n;
+      gemm_acc(Ab, XS + jt * WAVE);
       cx<R> old[E];
 #pragma unroll
       for (int el = 0; el < E; ++el) {
@@ -245,20 +301,19 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
         const int off = lo + (row - g) * nq;
         old[el] = (row < n && cin) ? epss[off] : cx<R>{0, 0};
       }
-      gemm_acc(Ab, XS + jt * WAVE);
 #pragma unroll
       for (int el = 0; el < E; ++el) {
         const int row = GJ::row_of(el, g);
         const int off = lo + (row - g) * nq;
         if (row < n && cin) epss[off] = old[el] + cx<R>{xr[el], xi[el]};
       }
+      gemm_acc(Aa, XS + (NT16 + jt) * WAVE);
 #pragma unroll
       for (int el = 0; el < E; ++el) {
         const int row = GJ::row_of(el, g);
         const int off = lo + (row - g) * nq;
         old[el] = (row < n && cin) ? eps[off] : cx<R>{0, 0};
       }
-      gemm_acc(Aa, XS + (NT16 + jt) * WAVE);
 #pragma unroll
       for (int el = 0; el < E; ++el) {
         const int row = GJ::row_of(el, g);
@@ -286,23 +341,29 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
     }
     if (threadIdx.x < N) L.gj.pinv[threadIdx.x] = threadIdx.x;
     __syncthreads();
-    // max_ij |alpha_ij| + |beta_ij|  (:205); the alpha owners read beta's fragments
-    double m = 0.0;
+    // Stop when max_ij |alpha_ij| + |beta_ij| < stop  (:205); the alpha owners
+    // read beta's fragments.  Only the decision is needed: a wave vote, with
+    // the hardware square root (approximate, but near the threshold the terms
+    // shrink by many orders of magnitude per iteration).  NaN keeps
+    // iterating, as the reference's comparison does.
+    bool go_on = false;
     if (kind == 1) {
 #pragma unroll
       for (int el = 0; el < E; ++el) {
         const int row = GJ::row_of(el, g);
-        if (row < n && col < n)
-          m = fmax(m, cabs(cx<R>{xr[el], xi[el]}) + cabs(Ab[fslot(el)]));
+        const cx<R> bv = Ab[fslot(el)];
+        const double sa = __builtin_amdgcn_sqrt(xr[el] * xr[el] + xi[el] * xi[el]);
+        const double sb = __builtin_amdgcn_sqrt(bv.re * bv.re + bv.im * bv.im);
+        if (row < n && col < n) go_on |= !(sa + sb < a.stop);
       }
     }
-    m = wave_max(m);
-    if (lane == 0) red[wave] = m;
+    go_on = __any(go_on);
+    if (lane == 0) red[wave] = go_on ? 1.0 : 0.0;
     __syncthreads();
-    m = red[0];
+    double anyw = 0.0;
 #pragma unroll
-    for (int w = 1; w < NW; ++w) m = fmax(m, red[w]);
-    if (m < a.stop) break;
+    for (int w = 0; w < NW; ++w) anyw += red[w];
+    if (anyw == 0.0) break;
     if (it >= 101) {
       status |= STATUS_LS_NOCONV;
       break;

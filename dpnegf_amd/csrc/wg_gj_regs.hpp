@@ -93,25 +93,54 @@ struct RegGJ {
   }
 
   // Sweeper, step 2: the four pivot steps of panel j0 on PAN, one row per
-  // lane.  `used`: this lane's row has pivoted in an earlier panel.  Pivot =
-  // largest |re| + |im| among unused rows, compared in single precision to 17
-  // mantissa bits, lowest row on ties (a DPP wave maximum).  Publishes -Fm
-  // (Fm: the multipliers, zero for a pivot row's own and earlier steps), the
-  // pivot rows, L, the reciprocals, piv / pinv and the singular flag.
+  // lane.  `used`: this lane's row has pivoted in an earlier panel.
   static __device__ __forceinline__ void sweep(int j0, int buf, bool used, Shared* sh) {
     const int lane = threadIdx.x & (WAVE - 1);
+    cx<double> v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = lane < N ? sh->PAN[lane][c] : cx<double>{0, 0};
+    sweep_row<4>(v, j0, buf, used, sh);
+  }
+
+  // The four pivot steps of panel j0 on a row held by this lane (lane = row):
+  // v[0..3] are the panel's columns, v[4..NC-1] further columns that receive
+  // the same row operations.  Pivot = largest |re| + |im| among unused rows,
+  // compared in single precision to 17 mantissa bits, lowest row on ties (a
+  // DPP wave maximum).
+  //
+  // Each step eliminates with the standard multiplier, row -= (f / d) prow,
+  // and leaves the pivot row unscaled; the pivot rows are scaled by 1 / d
+  // once at the end of the panel (an unscaled row r satisfies
+  // r = d * (the scaled row) through every later step, so the result is the
+  // Gauss-Jordan one).  Per column and step that is four FMAs.
+  //
+  // Publishes -Fm (Fm: the multipliers of the scaled rows, zero for a pivot
+  // row's own and earlier steps), the pivot rows, L (the pivot rows' values at
+  // earlier steps, never scaled: they were not pivots yet), the reciprocals,
+  // piv / pinv and the singular flag.
+  template <int NC>
+  static __device__ __forceinline__ void sweep_row(cx<double> (&v)[NC], int j0, int buf,
+                                                   bool used, Shared* sh) {
+    sweep_row<NC>([&](int c) { return v[c]; }, [&](int c, cx<double> x) { v[c] = x; }, j0, buf,
+                  used, sh);
+  }
+  // Same, with column c of the row read and written through get(c) /
+  // set(c, x) (c a compile-time constant after unrolling), so that a row held
+  // in the tile's own registers is swept in place.
+  template <int NC, class GET, class SET>
+  static __device__ __forceinline__ void sweep_row(GET get, SET set, int j0, int buf, bool used,
+                                                   Shared* sh) {
+    const int lane = threadIdx.x & (WAVE - 1);
     const bool row_ok = lane < N;
-    cx<double> v[4], F[4], dinv[4], L[4][4];
+    cx<double> F[4], dinv[4];
     int pr[4];
     int pstep = 4;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) v[c] = row_ok ? sh->PAN[lane][c] : cx<double>{0, 0};
     bool sing = false;
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
       unsigned key = 0;
       if (row_ok && !used) {
-        const float kf = (float)pivot_key(v[jj]);
+        const float kf = (float)pivot_key(get(jj));
         key = (((__float_as_uint(kf) >> 6) + 1u) << 6) | (unsigned)(WAVE - 1 - lane);
       }
       key = wave_max_u32(key);
@@ -119,27 +148,39 @@ struct RegGJ {
       p = p < N ? p : N - 1;
       pr[jj] = p;
       const bool isp = (lane == p);
-      if (isp) { used = true; pstep = jj; }
-      F[jj] = v[jj];
+      if (isp) {
+        used = true;
+        pstep = jj;
 #pragma unroll
-      for (int k2 = 0; k2 < jj; ++k2) L[jj][k2] = bcast_lane(F[k2], p);
-      cx<double> prow[4];
-#pragma unroll
-      for (int c = jj; c < 4; ++c) prow[c] = bcast_lane(v[c], p);
-      const cx<double> d = prow[jj];
+        for (int k2 = 0; k2 < jj; ++k2) sh->L[buf][jj][k2] = F[k2];
+      }
+      F[jj] = get(jj);
+      const cx<double> d = bcast_lane(F[jj], p);
       dinv[jj] = recip_nr(d);
       sing |= (d.re == 0.0 && d.im == 0.0);
-      const cx<double> fj = F[jj];
+      // multiplier f / d; the pivot row is left as it is
+      const cx<double> m0 = F[jj] * dinv[jj];
+      const cx<double> mj = {isp ? 0.0 : m0.re, isp ? 0.0 : m0.im};
 #pragma unroll
-      for (int c = jj + 1; c < 4; ++c) {
-        const cx<double> rc = prow[c] * dinv[jj];
-        v[c] = isp ? rc : v[c] - fj * rc;
+      for (int c = jj + 1; c < NC; ++c) {
+        const cx<double> vc = get(c);
+        const cx<double> pc = bcast_lane(vc, p);
+        set(c, cx<double>{fma(-mj.re, pc.re, fma(mj.im, pc.im, vc.re)),
+                          fma(-mj.re, pc.im, fma(-mj.im, pc.re, vc.im))});
       }
     }
+    // scale the pivot rows: by 1 / d of their step, for the further columns
+    // and for the multipliers of later steps
+    cx<double> s = {1.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (pstep == k) s = dinv[k];
+#pragma unroll
+    for (int c = 4; c < NC; ++c) set(c, get(c) * s);
     if (row_ok) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const cx<double> fm = (k <= pstep && pstep < 4) ? cx<double>{0, 0} : F[k];
+        const cx<double> fm = (pstep < 4 && k <= pstep) ? cx<double>{0, 0} : F[k] * s;
         sh->FM[buf][lane >> 4][(lane & 15) + 16 * k] = -fm;
       }
     }
@@ -150,10 +191,32 @@ struct RegGJ {
         sh->dinv[buf][k] = dinv[k];
         sh->piv[j0 + k] = pr[k];
         sh->pinv[pr[k]] = j0 + k;
-#pragma unroll
-        for (int k2 = 0; k2 < k; ++k2) sh->L[buf][k][k2] = L[k][k2];
       }
       if (sing) sh->singular = 1;
+    }
+  }
+
+  // The calling wave's tile (16 columns) to one row per lane through the
+  // 16 N-entry LDS buffer tb: v[0..3] = columns 0..3 of row `lane`, the
+  // other twelve columns into xr / xi (E >= 12), as (xr[m], xi[m]) = column
+  // 4 + m.  Lanes >= N get zeros.
+  static __device__ __forceinline__ void tile_to_rows(vE& xr, vE& xi, cx<double> (&v)[4],
+                                                      cx<double>* tb) {
+    static_assert(E >= 12, "row-per-lane tile needs twelve register pairs");
+    const int lane = threadIdx.x & (WAVE - 1), g = lane >> 4, c = lane & 15;
+#pragma unroll
+    for (int e = 0; e < E; ++e) tb[row_of(e, g) * 16 + c] = cx<double>{xr[e], xi[e]};
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const bool ok = lane < N;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = ok ? tb[lane * 16 + k] : cx<double>{0, 0};
+#pragma unroll
+    for (int m = 0; m < 12; ++m) {
+      const cx<double> t = ok ? tb[lane * 16 + 4 + m] : cx<double>{0, 0};
+      xr[m] = t.re;
+      xi[m] = t.im;
     }
   }
 
