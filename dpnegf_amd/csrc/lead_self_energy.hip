@@ -34,6 +34,8 @@
 //                        W = zS - eps for the next solve
 // eps lives in the workspace and eps_s in the g_s output, each read and
 // written by the same lanes every iteration.
+#include <type_traits>
+
 #include "rgf_kernels.hpp"
 #include "wg_gj_regs.hpp"
 
