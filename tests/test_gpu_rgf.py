@@ -176,6 +176,49 @@ def test_lead_self_energy_matches_oracle(name):
         check_surface(res, r)
 
 
+def _random_lead(n, nr, overlap, seed):
+    """Lead tuple (hL, hLL, hDL, sL, sLL, sDL) with dense Hermitian hL, dense
+    hLL and hDL (nr x n), and for `overlap` a Hermitian positive-definite sL
+    with small sLL, sDL; else the orthogonal pattern (I, 0, 0)."""
+    rng = np.random.default_rng(seed)
+
+    def rc(*shape):
+        return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+    a = rc(n, n)
+    hL = 0.25 * (a + a.conj().T)
+    hLL = 0.3 * rc(n, n)
+    hDL = 0.3 * rc(nr, n)
+    if overlap:
+        b = rc(n, n)
+        sL = np.eye(n) + 0.02 * (b + b.conj().T)
+        sLL, sDL = 0.02 * rc(n, n), 0.02 * rc(nr, n)
+    else:
+        sL, sLL, sDL = np.eye(n), np.zeros((n, n)), np.zeros((nr, n))
+    return tuple(torch.tensor(x, dtype=torch.complex128) for x in (hL, hLL, hDL, sL, sLL, sDL))
+
+
+@pytest.mark.parametrize("n,nr,overlap", [(24, 24, True), (32, 20, False), (40, 40, False),
+                                          (44, 30, True), (48, 48, True)])
+def test_lead_self_energy_sizes_and_overlap(n, nr, overlap):
+    """Every instantiation of the fused decimation (padded to 32 or 48, with
+    and without an overlap matrix) against the oracle."""
+    from dpnegf_amd.negf.surface_green import self_energy_batched
+    dev = _dev()
+    lead = _random_lead(n, nr, overlap, seed=n + nr)
+
+    class _S:  # the oracle helper takes an object with lead_L / lead_R
+        lead_L = lead_R = lead
+
+    E = np.linspace(-1.5, 1.5, 5)
+    print(f"case lead n={n} nr={nr} overlap={overlap}", flush=True)
+    ref = _leads_sigmas_oracle(_S, E, 1e-4)[0]
+    res = self_energy_batched([tuple(x.to(dev) for x in lead)], _f64(E), 1e-4,
+                              raise_on_error=False)
+    assert int((res.status & 7).sum().item()) == 0, res.status
+    check_surface(res, ref)
+
+
 # Points of the 4096-energy grid linspace(-3, 3, 4096) where the W=48 ribbon
 # lead's decimation is worst conditioned at eta = 1e-6 (Dyson residual of an
 # explicit-inverse decimation above 1e-3 there).
