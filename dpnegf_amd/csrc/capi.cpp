@@ -267,6 +267,21 @@ int64_t dpnegf_rgf_transmission_blocked_workspace_elems(int K, const int32_t* h_
   return negf::rgf_transmission_ops_workspace(K, h_bsz, B);
 }
 
+int64_t dpnegf_autotune_query(int K, const int32_t* h_bsz, int64_t mem_bytes) {
+  if (!h_bsz || K < 1 || mem_bytes < 0) return fail(DPNEGF_ERR_ARG, "bad arguments");
+  for (int q = 0; q < K; ++q)
+    if (h_bsz[q] < 1) return fail(DPNEGF_ERR_ARG, "block %d has size %d", q, h_bsz[q]);
+  // workspace plus z, Sigma_L, Sigma_R, T and status per point, linear in B
+  const int64_t w1 = negf::rgf_transmission_ops_workspace(K, h_bsz, 1);
+  const int64_t w2 = negf::rgf_transmission_ops_workspace(K, h_bsz, 2);
+  const int64_t n0 = h_bsz[0], nl = h_bsz[K - 1];
+  const int64_t per = (w2 - w1) * 16 + (1 + n0 * n0 + nl * nl) * 16 + 8 + 4;
+  const int64_t fixed = w1 * 16 - (w2 - w1) * 16;
+  int64_t b = (mem_bytes - fixed) / per;
+  if (b < 0) b = 0;
+  return b > 65535 ? 65535 : b;
+}
+
 int dpnegf_rgf_transmission_blocked(void* stream, int K, const int32_t* h_bsz, const void* arena,
                                     int B, const void* z, const void* sigL, int nL,
                                     const void* sigR, int nR, int ortho, double* T, void* work,

@@ -12,6 +12,7 @@ Inputs must be CUDA (HIP) tensors; there is no CPU path.
 import torch
 
 from .. import _native as nat
+from . import autotune
 from ._btd import (C128, block_sizes, corner_self_energy, is_orthogonal, pack_arena,
                    pack_batched_blocks, split_blocks)
 
@@ -176,10 +177,14 @@ class BTDSystems:
         self._h_bsz = nat.host_i32(self.sizes)
 
 
-def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", status=None, out=None):
+def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", status=None, out=None,
+                     chunk=None):
     """T for every point: ``z`` [P] complex (E + E_ref + i eta), ``sigL`` [P, nL, nL],
     ``sigR`` [P, nR, nR] (already trimmed to their corners), ``sys_idx`` [P] int32
-    (k-point of each point).  Returns T [P] float64 (and fills ``status``)."""
+    (k-point of each point).  Returns T [P] float64 (and fills ``status``).
+    ``chunk``: points per call of the block-operation sweep (blocks above 64);
+    by default ``autotune.DEFAULT_CHUNK``, or ``autotune.blocked_chunk``'s pick
+    when ``DPNEGF_AUTOTUNE=1``."""
     if not isinstance(systems, BTDSystems):
         raise TypeError("systems must be a BTDSystems")
     dev = systems.device
@@ -197,7 +202,8 @@ def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", sta
         raise NotImplementedError("mixed precision needs block sizes <= 64")
     K = len(systems.sizes)
     idx = (sys_idx.cpu() if sys_idx is not None else torch.zeros(P, dtype=torch.int32))
-    chunk = 4096
+    if chunk is None:
+        chunk = autotune.blocked_chunk(systems, P) if autotune.enabled() else autotune.DEFAULT_CHUNK
     for s in range(systems.nsys):
         pts = (idx == s).nonzero().reshape(-1)
         if pts.numel() == 0:

@@ -95,6 +95,14 @@ int dpnegf_lead_self_energy(void* stream, int n, int nred, int mode, const void*
 /* Complex elements of the work buffer dpnegf_rgf_transmission_blocked needs. */
 int64_t dpnegf_rgf_transmission_blocked_workspace_elems(int K, const int32_t* h_bsz, int B);
 
+/* Batch shapes for the autotuner: the largest number of points B per
+ * dpnegf_rgf_transmission_blocked call whose workspace and per-point buffers
+ * (z, Sigma_L and Sigma_R at the corner block sizes, T, status) fit in
+ * mem_bytes, capped at 65535; negative on bad arguments.  No device work.
+ * The reference has no counterpart: its runner splits the energy grid into
+ * chunks of a user-set size (NEGF.py:602-631). */
+int64_t dpnegf_autotune_query(int K, const int32_t* h_bsz, int64_t mem_bytes);
+
 /* Transmission of B energies of one system for block sizes up to
  * DPNEGF_FULL_NMAX (the fused kernel stops at 64): the same sweep driven block
  * by block from the host with batched products and inverses.  Same result

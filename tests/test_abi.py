@@ -44,3 +44,35 @@ def test_argument_errors_are_reported_without_a_device():
     with pytest.raises(ValueError):
         nat.call("dpnegf_rgf_full", None, 0, None, None, 1, None, None, 0, None, 0, 1,
                  *([None] * 17))
+
+
+@pytest.mark.skipif(not os.path.exists(nat.lib_path()), reason="library not built")
+def test_autotune_query_fits_the_workspace():
+    """The largest batch the query returns fits the budget and the next one
+    does not (workspace plus z, both self-energies, T and status per point)."""
+    lib = nat.load()
+    for sizes in ([72, 80, 96], [168, 120, 144, 96], [256] * 4):
+        h = nat.host_i32(sizes)
+        K = len(sizes)
+
+        def need(B):
+            w = lib.dpnegf_rgf_transmission_blocked_workspace_elems(K, h, B) * 16
+            return w + B * ((1 + sizes[0] ** 2 + sizes[-1] ** 2) * 16 + 8 + 4)
+
+        for mem in (10 ** 7, 3 * 10 ** 8, 5 * 10 ** 9):
+            b = lib.dpnegf_autotune_query(K, h, mem)
+            assert 0 <= b <= 65535
+            assert need(b) <= mem
+            if b < 65535:
+                assert need(b + 1) > mem
+    assert lib.dpnegf_autotune_query(0, nat.host_i32([1]), 10 ** 6) < 0
+
+
+def test_every_package_module_imports():
+    """Import errors surface on the CPU, not first on a GPU host."""
+    import importlib
+    import pkgutil
+
+    import dpnegf_amd
+    for m in pkgutil.walk_packages(dpnegf_amd.__path__, "dpnegf_amd."):
+        importlib.import_module(m.name)
