@@ -26,19 +26,6 @@
 
 namespace negf {
 
-// 1/d = conj(d) / |d|^2, with 1/|d|^2 from the hardware reciprocal refined by
-// two Newton steps (a few ulp; shorter dependency chain than the IEEE division
-// sequence).  Pivots are far from the overflow / underflow range of |d|^2.
-__device__ __forceinline__ cx<double> recip_nr(cx<double> d) {
-  const double m = d.re * d.re + d.im * d.im;
-  double r = __builtin_amdgcn_rcp(m);
-  double e = fma(-m, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-m, r, 1.0);
-  r = fma(r, e, r);
-  return {d.re * r, -d.im * r};
-}
-
 template <int N>
 struct RegGJ {
   static_assert(N % 16 == 0 && N <= WAVE, "register Gauss-Jordan: N multiple of 16, <= 64");

@@ -159,6 +159,26 @@ __device__ __forceinline__ cx<R> recip_fast(cx<R> d) {
   return {d.re * s, -d.im * s};
 }
 
+// 1/d = conj(d) / |d|^2, with 1/|d|^2 from the hardware reciprocal refined by
+// Newton steps (two in FP64, one in FP32: a few ulp, and a shorter dependency
+// chain than the IEEE division sequence).  Pivots are far from the overflow /
+// underflow range of |d|^2.
+__device__ __forceinline__ cx<double> recip_nr(cx<double> d) {
+  const double m = d.re * d.re + d.im * d.im;
+  double r = __builtin_amdgcn_rcp(m);
+  double e = fma(-m, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-m, r, 1.0);
+  r = fma(r, e, r);
+  return {d.re * r, -d.im * r};
+}
+__device__ __forceinline__ cx<float> recip_nr(cx<float> d) {
+  const float m = d.re * d.re + d.im * d.im;
+  float r = __builtin_amdgcn_rcpf(m);
+  r = fmaf(r, fmaf(-m, r, 1.0f), r);
+  return {d.re * r, -d.im * r};
+}
+
 // Pivot key: LAPACK izamax's |re| + |im|; NaN counts as 0 so that a row is
 // always selectable (the selection below never returns an out-of-range row).
 template <typename R>
@@ -314,7 +334,7 @@ __device__ __forceinline__ int wg_gj_inverse(int n, FL load, FS store, GJScratch
 // ---- blocked Gauss-Jordan (matrix in LDS) -------------------------------------
 template <typename R, int N>
 struct GJBScratch {
-  cx<R> F[N][4];  // rank-4 multipliers of the current panel (inverse only)
+  cx<R> F[N][4];  // rank-4 multipliers of the current panel
   int pr[4];      // pivot rows of the current panel
   int piv[N];     // piv[j]: row pivoting column j
   int pinv[N];
@@ -335,16 +355,21 @@ __device__ __forceinline__ cx<R> bcast_lane(cx<R> x, int src) {
   return {bcast_lane(x.re, src), bcast_lane(x.im, src)};
 }
 
-// Blocked Gauss-Jordan on the N x NCOL matrix A in LDS (leading dimension LDA),
-// N <= 64 and a multiple of 4, workgroup of NT threads; rows are never
-// swapped (piv[j] is the row that pivots column j).
+// Blocked Gauss-Jordan in-place inverse of the N x N matrix A in LDS (leading
+// dimension LDA), N <= 64 and a multiple of 4, workgroup of NT threads; rows
+// are never swapped (piv[j] is the row that pivots column j).
 //
-// Panels of BW = 4 columns.  Wave 0 sweeps a panel with exactly the
-// sequential algorithm (one row per lane; pivot = largest |re| + |im| among
-// unused rows, compared in single precision to 17 mantissa bits, lowest row on
-// ties, found with a DPP wave maximum; pivot row broadcast with readlane),
-// recording each row's multiplier F[i][k] (its column-k value at step k).  For
-// the other columns C the same steps are, exactly as in sequential order:
+// Panels of BW = 4 columns.  Wave 0 sweeps a panel one row per lane (pivot =
+// largest |re| + |im| among unused rows, compared in single precision to 17
+// mantissa bits, lowest row on ties, found with a DPP wave maximum; pivot row
+// broadcast with readlane), recording each row's multiplier F[i][k] (its
+// column-k value at step k).  Each step eliminates with the standard
+// multiplier, row -= (f / d) prow, the panel column taking 1 in the pivot
+// row and -f / d elsewhere, and leaves the pivot row unscaled; the pivot rows
+// are scaled by 1 / d once at the end of the panel (an unscaled row stays d
+// times the scaled one through every later step), and so are their
+// multipliers of later steps.  For the other columns C the same steps are,
+// exactly as in sequential order:
 //   pivot rows  u_k = (Z_k - sum_{k'<k} F[p_k][k'] u_k') / d_k   (Z: rows before
 //               the panel; forward substitution by wave 0, written over Z)
 //   all rows    A[i, C] -= sum_k Fm[i][k] u_k   (rank-4, matrix cores)
@@ -352,18 +377,12 @@ __device__ __forceinline__ cx<R> bcast_lane(cx<R> x, int src) {
 // This is the blocked-LU form of the update; applying the panel's combined
 // transform to the unreduced pivot rows instead (an inverse of the 4 x 4 pivot
 // block times Z) costs about two digits at ill-conditioned points.
-//
-// INV: in-place inverse of the N x N matrix (NCOL = N).  Panel columns keep
-//      the in-place inverse values, C = every other column, F lives in sh->F.
-// !INV (solve [W | B], NCOL = N + NC): C = columns right of the panel (swept
-//      columns of W are never needed again) and F is written over the dead
-//      panel columns.
 // Returns nonzero (uniformly) if an exactly-zero pivot was met.
-template <typename R, int NT, int N, int NCOL, int LDA, bool INV>
+template <typename R, int NT, int N, int LDA>
 __device__ __forceinline__ int wg_gj_blocked_core(cx<R>* A, GJBScratch<R, N>* sh) {
   constexpr int BW = 4;
   constexpr int NW = NT / WAVE;
-  constexpr int NTILE = ((N + 15) / 16) * ((NCOL - (INV ? 0 : BW) + 15) / 16);
+  constexpr int NTILE = ((N + 15) / 16) * ((N + 15) / 16);
   constexpr int MAXT = (NTILE + NW - 1) / NW;
   static_assert(N <= WAVE && N % BW == 0, "blocked Gauss-Jordan: N <= 64, multiple of 4");
   const int t = threadIdx.x;
@@ -395,48 +414,38 @@ __device__ __forceinline__ int wg_gj_blocked_core(cx<R>* A, GJBScratch<R, N>* sh
         F[jj] = v[jj];
 #pragma unroll
         for (int k2 = 0; k2 < jj; ++k2) L[jj][k2] = bcast_lane(F[k2], p);
-        // The solve needs only the columns right of jj (the panel's own values
-        // are replaced by the multipliers); the inverse keeps all four.
-        cx<R> prow[BW];
-#pragma unroll
-        for (int c = 0; c < BW; ++c)
-          if (INV || c >= jj) prow[c] = bcast_lane(v[c], p);
-        const cx<R> d = prow[jj];
-        dinv[jj] = recip_fast(d);
+        const cx<R> d = bcast_lane(F[jj], p);
+        dinv[jj] = recip_nr(d);
         if (lane == 0) {
           sh->piv[j0 + jj] = p;
           sh->pr[jj] = p;
           if (d.re == R(0) && d.im == R(0)) sh->singular = 1;
         }
-        const cx<R> fj = F[jj];
+        const cx<R> m0 = F[jj] * dinv[jj];
+        const cx<R> m = {isp ? R(0) : m0.re, isp ? R(0) : m0.im};
 #pragma unroll
         for (int c = 0; c < BW; ++c) {
-          if (c == jj) {
-            if (INV) v[c] = isp ? dinv[jj] : -(fj * dinv[jj]);
-          } else if (INV || c > jj) {
-            const cx<R> rc = prow[c] * dinv[jj];
-            v[c] = isp ? rc : v[c] - fj * rc;
-          }
+          if (c == jj) continue;
+          const cx<R> pc = bcast_lane(v[c], p);
+          v[c].re = fma(-m.re, pc.re, fma(m.im, pc.im, v[c].re));
+          v[c].im = fma(-m.re, pc.im, fma(-m.im, pc.re, v[c].im));
         }
+        v[jj] = cx<R>{isp ? R(1) : -m.re, isp ? R(0) : -m.im};
       }
-      cx<R> Fm[BW];
+      cx<R> s = {R(1), R(0)};
 #pragma unroll
-      for (int k = 0; k < BW; ++k) Fm[k] = (k <= pstep && pstep < BW) ? cx<R>{0, 0} : F[k];
+      for (int k = 0; k < BW; ++k)
+        if (pstep == k) s = dinv[k];
       if (row_ok) {
 #pragma unroll
         for (int c = 0; c < BW; ++c) {
-          if (INV) {
-            A[lane * LDA + j0 + c] = v[c];
-            sh->F[lane][c] = Fm[c];
-          } else {
-            A[lane * LDA + j0 + c] = Fm[c];
-          }
+          A[lane * LDA + j0 + c] = v[c] * s;
+          sh->F[lane][c] = (pstep < BW && c <= pstep) ? cx<R>{0, 0} : F[c] * s;
         }
       }
       // pivot rows of the other columns: forward substitution in step order
-      const int clo = INV ? 0 : j0 + BW;
-      for (int c = clo + lane; c < NCOL; c += WAVE) {
-        if (INV && c >= j0 && c < j0 + BW) continue;
+      for (int c = lane; c < N; c += WAVE) {
+        if (c >= j0 && c < j0 + BW) continue;
         cx<R> u[BW];
 #pragma unroll
         for (int k = 0; k < BW; ++k) {
@@ -450,18 +459,15 @@ __device__ __forceinline__ int wg_gj_blocked_core(cx<R>* A, GJBScratch<R, N>* sh
       }
     }
     __syncthreads();
-    const int c0 = INV ? 0 : j0 + BW;
     const int p0 = sh->pr[0], p1 = sh->pr[1], p2 = sh->pr[2], p3 = sh->pr[3];
     wg_gemm<R, NW, MAXT, true, 1>(
-        N, NCOL - c0, BW,
-        [&](int i, int k) { return INV ? sh->F[i][k] : A[i * LDA + j0 + k]; },
+        N, N, BW, [&](int i, int k) { return sh->F[i][k]; },
         [&](int k, int c) {
           const int pk = k == 0 ? p0 : k == 1 ? p1 : k == 2 ? p2 : p3;
-          return A[pk * LDA + c0 + c];
+          return A[pk * LDA + c];
         },
         [&](int i, int c, cx<R> val) {
-          const int cc = c0 + c;
-          if (!INV || cc < j0 || cc >= j0 + BW) A[i * LDA + cc] = A[i * LDA + cc] - val;
+          if (c < j0 || c >= j0 + BW) A[i * LDA + c] = A[i * LDA + c] - val;
         });
     __syncthreads();
   }
@@ -475,7 +481,7 @@ __device__ __forceinline__ int wg_gj_blocked_core(cx<R>* A, GJBScratch<R, N>* sh
 template <typename R, int NT, int N, int LDA, class FS>
 __device__ __forceinline__ int wg_gj_inverse_blocked(cx<R>* A, FS store, GJBScratch<R, N>* sh) {
   const int t = threadIdx.x;
-  const int sing = wg_gj_blocked_core<R, NT, N, N, LDA, true>(A, sh);
+  const int sing = wg_gj_blocked_core<R, NT, N, LDA>(A, sh);
   for (int j = t; j < N; j += NT) sh->pinv[j] = j;
   __syncthreads();
   for (int j = t; j < N; j += NT) {
@@ -498,17 +504,6 @@ __device__ __forceinline__ int wg_gj_inverse_blocked(cx<R>* A, FS store, GJBScra
   }
   __syncthreads();
   return sing;
-}
-
-// Solves W X = B by Gauss-Jordan elimination on the augmented A = [W | B]
-// (LDS, N x (N + NC), leading dimension LDA) with the blocked update above.
-// On return row piv[j] of the B columns holds row j of X:
-// X(j, c) = A[piv[j] * LDA + N + c].  Elimination on [W | B] is forward stable
-// like an LU solve with the stacked right-hand sides, which W^-1 B through an
-// explicit inverse is not.  Returns nonzero if an exactly-zero pivot was met.
-template <typename R, int NT, int N, int NC, int LDA>
-__device__ __forceinline__ int wg_gj_solve_blocked(cx<R>* A, GJBScratch<R, N>* sh) {
-  return wg_gj_blocked_core<R, NT, N, N + NC, LDA, false>(A, sh);
 }
 
 }  // namespace negf
