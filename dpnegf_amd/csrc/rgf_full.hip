@@ -42,64 +42,102 @@ __device__ __forceinline__ cx<double> view_at(const OpView& o, int b, int i, int
   return {v.re, o.csign * v.im};
 }
 
-// 64 x 64 output tile per workgroup; wave w owns the 2 x 2 block of 16 x 16
-// tiles at rows 32 (w / 2), columns 32 (w % 2), so each k-step loads two A and
-// two B values per lane.  Indices past the edges are clamped (every load is in
-// bounds) and their values zeroed, so the k-loop has no branches, and the next
-// k-step's operands are loaded before this step's matrix-core instructions.
+// Batched C = alpha op(A) op(B) + beta C: a 64 x 64 output tile per
+// workgroup, wave w owning the 2 x 2 block of 16 x 16 tiles at rows 32 (w / 2),
+// columns 32 (w % 2).  K is staged through LDS in chunks of KC = 16: the 256
+// threads load the A chunk (64 x 16, thread t at column k = t % 16 of rows
+// t / 16 + 16 u) and the B chunk (16 x 64, row t / 16, columns t % 16 + 16 u),
+// so sixteen consecutive lanes read sixteen consecutive elements; the operand
+// views are applied on load, indices clamped (every load is in bounds) and
+// out-of-range values zeroed (a chunk past K contributes zeros).  Double buffered: the next
+// chunk's loads are in flight during this chunk's matrix-core steps.  A's LDS
+// rows are padded to 17 elements so that the fragment reads (16 rows at one k)
+// spread over the banks.
 template <bool SA, bool SB>
-__global__ __launch_bounds__(256) void bgemm_kernel(BGemmArgs a) {
+__global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
   typedef mfma<double>::v4 v4;
+  constexpr int KC = 16, LDAS = KC + 1, LDBS = 64;
+  __shared__ cx<double> As[2][64 * LDAS];
+  __shared__ cx<double> Bs[2][KC * LDBS];
   const int tn = (a.n + 63) / 64;
   const int ti = blockIdx.x / tn, tj = blockIdx.x % tn;
   const int b = blockIdx.y;
   const int i0 = ti * 64, j0 = tj * 64;
   const cx<double> zb = a.z ? a.z[b] : cx<double>{0, 0};
   const OpView A = op_view(a.A), Bv = op_view(a.B_);
-  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const int t = threadIdx.x;
+  const int lane = t & (WAVE - 1), w = t / WAVE;
   const int li = lane & 15, lk = lane >> 4;
-  int ri[2], cj[2];
-  bool rok[2], cok[2];
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int r = i0 + 32 * (w / 2) + 16 * t + li, c = j0 + 32 * (w % 2) + 16 * t + li;
-    rok[t] = r < a.m; cok[t] = c < a.n;
-    ri[t] = rok[t] ? r : a.m - 1;
-    cj[t] = cok[t] ? c : a.n - 1;
-  }
   const int K = a.k;
-  auto ld_a = [&](int t, int k) -> cx<double> {
-    const bool ok = rok[t] && k < K;
-    const cx<double> v = view_at<SA>(A, b, ri[t], k < K ? k : K - 1, zb);
-    return {ok ? v.re : 0.0, ok ? v.im : 0.0};
+  // load assignment
+  const int ak = t & 15, ar = t >> 4;  // A: column ak of rows ar + 16 u
+  const int bk = t >> 4, bc = t & 15;  // B: row bk, columns bc + 16 u
+  int arow[4], bcol[4];
+  bool arok[4], bcok[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int r = i0 + ar + 16 * u, c = j0 + bc + 16 * u;
+    arok[u] = r < a.m;
+    bcok[u] = c < a.n;
+    arow[u] = arok[u] ? r : a.m - 1;
+    bcol[u] = bcok[u] ? c : a.n - 1;
+  }
+  cx<double> ra[4], rb[4];
+  auto load_chunk = [&](int k0) __attribute__((always_inline)) {
+    const int ka = k0 + ak, kb = k0 + bk;
+    const bool kaok = ka < K, kbok = kb < K;
+    const int kac = kaok ? ka : K - 1, kbc = kbok ? kb : K - 1;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const cx<double> va = view_at<SA>(A, b, arow[u], kac, zb);
+      const bool oa = arok[u] && kaok;
+      ra[u] = {oa ? va.re : 0.0, oa ? va.im : 0.0};
+      const cx<double> vb = view_at<SB>(Bv, b, kbc, bcol[u], zb);
+      const bool ob = bcok[u] && kbok;
+      rb[u] = {ob ? vb.re : 0.0, ob ? vb.im : 0.0};
+    }
   };
-  auto ld_b = [&](int t, int k) -> cx<double> {
-    const bool ok = cok[t] && k < K;
-    const cx<double> v = view_at<SB>(Bv, b, k < K ? k : K - 1, cj[t], zb);
-    return {ok ? v.re : 0.0, ok ? v.im : 0.0};
+  auto store_chunk = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      As[buf][(ar + 16 * u) * LDAS + ak] = ra[u];
+      Bs[buf][bk * LDBS + bc + 16 * u] = rb[u];
+    }
   };
   v4 cr[2][2], ci[2][2];
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y) cr[x][y] = ci[x][y] = v4{0, 0, 0, 0};
-  cx<double> pa[2] = {ld_a(0, lk), ld_a(1, lk)};
-  cx<double> pb[2] = {ld_b(0, lk), ld_b(1, lk)};
-  for (int k0 = 0; k0 < K; k0 += 4) {
-    const int kn = k0 + 4 + lk;
-    const cx<double> na[2] = {ld_a(0, kn), ld_a(1, kn)};
-    const cx<double> nb[2] = {ld_b(0, kn), ld_b(1, kn)};
+  const int rbase = 32 * (w / 2), cbase = 32 * (w % 2);
+  const int nchunks = (K + KC - 1) / KC;
+  load_chunk(0);
+  store_chunk(0);
+  __syncthreads();
+  for (int ch = 0; ch < nchunks; ++ch) {
+    const int buf = ch & 1;
+    const bool more = ch + 1 < nchunks;
+    if (more) load_chunk((ch + 1) * KC);
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int kk = 0; kk < KC / 4; ++kk) {
+      const int kl = 4 * kk + lk;
+      cx<double> pa[2], pb[2];
 #pragma unroll
-      for (int y = 0; y < 2; ++y) {
-        cr[x][y] = mfma<double>::op(pa[x].re, pb[y].re, cr[x][y]);
-        cr[x][y] = mfma<double>::op(-pa[x].im, pb[y].im, cr[x][y]);
-        ci[x][y] = mfma<double>::op(pa[x].re, pb[y].im, ci[x][y]);
-        ci[x][y] = mfma<double>::op(pa[x].im, pb[y].re, ci[x][y]);
-      }
-    pa[0] = na[0]; pa[1] = na[1];
-    pb[0] = nb[0]; pb[1] = nb[1];
+      for (int x = 0; x < 2; ++x) pa[x] = As[buf][(rbase + 16 * x + li) * LDAS + kl];
+#pragma unroll
+      for (int y = 0; y < 2; ++y) pb[y] = Bs[buf][kl * LDBS + cbase + 16 * y + li];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+          cr[x][y] = mfma<double>::op(pa[x].re, pb[y].re, cr[x][y]);
+          cr[x][y] = mfma<double>::op(-pa[x].im, pb[y].im, cr[x][y]);
+          ci[x][y] = mfma<double>::op(pa[x].re, pb[y].im, ci[x][y]);
+          ci[x][y] = mfma<double>::op(pa[x].im, pb[y].re, ci[x][y]);
+        }
+    }
+    if (more) store_chunk(buf ^ 1);
+    __syncthreads();
   }
   cx<double>* C = a.C + (int64_t)b * a.cstride;
   const cx<double> alpha = a.alpha, beta = a.beta;
@@ -108,8 +146,8 @@ __global__ __launch_bounds__(256) void bgemm_kernel(BGemmArgs a) {
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y) {
-      const int ib = i0 + 32 * (w / 2) + 16 * x;
-      const int j = j0 + 32 * (w % 2) + 16 * y + li;
+      const int ib = i0 + rbase + 16 * x;
+      const int j = j0 + cbase + 16 * y + li;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = ib + mfma<double>::row(lk, r);
@@ -457,10 +495,10 @@ static hipError_t gemm(hipStream_t st, int B, int m, int n, int k, Opnd A, Opnd 
   a.C = C; a.cstride = cstride; a.ldc = ldc; a.z = z;
   dim3 grid(((m + 63) / 64) * ((n + 63) / 64), B);
   const bool sa = A.s != nullptr, sb = Bo.s != nullptr;
-  if (sa && sb) hipLaunchKernelGGL((bgemm_kernel<true, true>), grid, dim3(256), 0, st, a);
-  else if (sa) hipLaunchKernelGGL((bgemm_kernel<true, false>), grid, dim3(256), 0, st, a);
-  else if (sb) hipLaunchKernelGGL((bgemm_kernel<false, true>), grid, dim3(256), 0, st, a);
-  else hipLaunchKernelGGL((bgemm_kernel<false, false>), grid, dim3(256), 0, st, a);
+  if (sa && sb) hipLaunchKernelGGL((bgemm_lds_kernel<true, true>), grid, dim3(256), 0, st, a);
+  else if (sa) hipLaunchKernelGGL((bgemm_lds_kernel<true, false>), grid, dim3(256), 0, st, a);
+  else if (sb) hipLaunchKernelGGL((bgemm_lds_kernel<false, true>), grid, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL((bgemm_lds_kernel<false, false>), grid, dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
