@@ -9,7 +9,15 @@ One step = the whole job for one energy grid with the Hamiltonian already in
 HBM: Lopez-Sancho self-energies of both leads at all 4096 energies, then the
 fused RGF transmission sweep.  ``value`` = 4096 / step time.
 
+``--workload multiorbital`` measures the uneven large-block device instead
+(3 orbitals/site, layer widths cycling 32/40/48/56 sites = blocks 96-168,
+40 layers, ``--kpoints`` transverse k-points x 1024 energies, eta = 1e-5):
+the same step (both leads, then the transmission sweep, here through the
+block-operation path for blocks above 64).  Its rate is per (E,k) point, so a
+few k-points give the same figure as 32; it is not the headline line.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
+                       [--workload ribbon48|multiorbital] [--kpoints NK]
 """
 import argparse
 import json
@@ -100,6 +108,84 @@ def cpu_baseline(s, E, eta, n_sample):
                       f"(scipy Lopez-Sancho) + RGF (torch) + T per energy, {dt:.1f} s"}
 
 
+def bench_multiorbital(args):
+    """Blocks 96/120/144/168, 40 layers, args.kpoints k-points x 1024 energies."""
+    from dpnegf_amd import workloads as wl
+    from dpnegf_amd.negf.recursive_green_cal import BTDSystems, rgf_transmission
+    from dpnegf_amd.negf.surface_green import self_energy_batched
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    nk, nE, eta = args.kpoints, 1024, 1e-5
+    systems = [wl.multiorbital(widths=(32, 40, 48, 56), K=40, norb=3, kpoint=k / 32, seed=0).to(dev)
+               for k in range(nk)]
+    E = torch.linspace(-2.0, 2.0, nE, dtype=torch.float64, device=dev)
+    e_all = E.repeat(nk).to(torch.complex128).contiguous()
+    sys_idx = torch.arange(nk, dtype=torch.int32, device=dev).repeat_interleave(nE).contiguous()
+    z = (e_all + 1j * eta).contiguous()
+    bs = BTDSystems([(s.hd, s.hu, s.hl, s.sd, s.su, s.sl) for s in systems], dev)
+    P = nk * nE
+    T = torch.empty(P, dtype=torch.float64, device=dev)
+    status = torch.zeros(P, dtype=torch.int32, device=dev)
+    res = {}
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record()
+        res["L"] = self_energy_batched([s.lead_L for s in systems], e_all, eta, sys_idx=sys_idx,
+                                       raise_on_error=False)
+        if ev is not None:
+            ev[1].record()
+        res["R"] = self_energy_batched([s.lead_R for s in systems], e_all, eta, sys_idx=sys_idx,
+                                       raise_on_error=False)
+        if ev is not None:
+            ev[2].record()
+        rgf_transmission(bs, z, res["L"].sigma, res["R"].sigma, sys_idx=sys_idx, status=status,
+                         out=T)
+        if ev is not None:
+            ev[3].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(4))
+              for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(events[k])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    leg = [float(np.mean([ev[i].elapsed_time(ev[i + 1]) for ev in events])) for i in range(3)]
+    nLd, nRd = bs.sizes[0], bs.sizes[-1]
+    rgf_flops = rgf_flops_per_point(bs.sizes, nLd, nRd)
+    lead_flops = [ls_flops_per_point(int(systems[0].lead_L[0].shape[0]),
+                                     float(res["L"].iters.double().mean().item())),
+                  ls_flops_per_point(int(systems[0].lead_R[0].shape[0]),
+                                     float(res["R"].iters.double().mean().item()))]
+    out = {
+        "metric": METRIC,
+        "value": round(P / (dt / args.steps), 3),
+        "unit": "(E,k) points/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"multiorbital 3 orb/site, widths 32/40/48/56 (blocks 96-168), "
+                               f"40 layers, seed 0, {nk} k-points x {nE} energies in [-2,2] eV, "
+                               f"eta=1e-5; step = Sigma_L + Sigma_R (Lopez-Sancho) + block-"
+                               f"operation RGF transmission", "points_per_step": P},
+        "lead_L_ms": round(leg[0], 3), "lead_R_ms": round(leg[1], 3), "rgf_ms": round(leg[2], 3),
+        "rgf_fp64_frac": rgf_flops * P / (leg[2] * 1e-3) / PEAK_FP64,
+        "lead_L_fp64_frac": lead_flops[0] * P / (leg[0] * 1e-3) / PEAK_FP64,
+        "lead_R_fp64_frac": lead_flops[1] * P / (leg[1] * 1e-3) / PEAK_FP64,
+        "lead_status_nonzero": int((res["L"].status != 0).sum().item()
+                                   + (res["R"].status != 0).sum().item()),
+        "T_checksum": float(T.sum().item()),
+        "status_nonzero": int((status != 0).sum().item()),
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +195,9 @@ def main():
     ap.add_argument("--energies", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="ribbon48", choices=["ribbon48", "multiorbital"])
+    ap.add_argument("--kpoints", type=int, default=32,
+                    help="transverse k-points of the multiorbital workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -118,6 +207,9 @@ def main():
               "the (E,k) points shard trivially across GPUs (see DESIGN.md); run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "multiorbital":
+        bench_multiorbital(args)
+        return
 
     from dpnegf_amd.negf.recursive_green_cal import BTDSystems, rgf_transmission
     from dpnegf_amd.negf.surface_green import self_energy_batched

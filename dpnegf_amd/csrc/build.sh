@@ -11,10 +11,18 @@ pids=()
 for f in rgf_transmission.hip lead_self_energy.hip rgf_full.hip capi.cpp; do
   o="$bld/${f%.*}.o"
   rm -f "$o"
+  extra=()
+  # The batched product's accumulators are loop-carried across a runtime K
+  # loop; in the default accumulator-register form the compiler copies all of
+  # them between register files every iteration.  No other kernel in this file
+  # uses the matrix cores.
+  if [[ "$f" == rgf_full.hip ]]; then
+    extra=(-mllvm -amdgpu-mfma-vgpr-form)
+  fi
   if [[ "$f" == *.cpp ]]; then
     hipcc $FLAGS -x hip -c "$here/$f" -o "$o" &
   else
-    hipcc $FLAGS -c "$here/$f" -o "$o" &
+    hipcc $FLAGS "${extra[@]}" -c "$here/$f" -o "$o" &
   fi
   pids+=($!)
   objs+=("$o")

@@ -14,6 +14,7 @@
 // bbuild (-(H - z_b S) - Sigma corners).
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "rgf_kernels.hpp"
@@ -34,25 +35,22 @@ __device__ __forceinline__ OpView op_view(const Opnd& o) {
   return {o.h, o.s, o.bstride, o.sstride, tr ? 1 : (int64_t)o.ld, tr ? (int64_t)o.ld : 1,
           (o.op == OP_H || o.op == OP_C) ? -1.0 : 1.0};
 }
-template <bool HAS_S>
-__device__ __forceinline__ cx<double> view_at(const OpView& o, int b, int i, int k, cx<double> zb) {
-  const int64_t off = (int64_t)i * o.rs + (int64_t)k * o.cs;
-  cx<double> v = o.h[(int64_t)b * o.bstride + off];
-  if (HAS_S) v = v - zb * o.s[(int64_t)b * o.sstride + off];
-  return {v.re, o.csign * v.im};
-}
 
 // Batched C = alpha op(A) op(B) + beta C: a 64 x 64 output tile per
 // workgroup, wave w owning the 2 x 2 block of 16 x 16 tiles at rows 32 (w / 2),
 // columns 32 (w % 2).  K is staged through LDS in chunks of KC = 16: the 256
 // threads load the A chunk (64 x 16, thread t at column k = t % 16 of rows
 // t / 16 + 16 u) and the B chunk (16 x 64, row t / 16, columns t % 16 + 16 u),
-// so sixteen consecutive lanes read sixteen consecutive elements; the operand
-// views are applied on load, indices clamped (every load is in bounds) and
-// out-of-range values zeroed (a chunk past K contributes zeros).  Double buffered: the next
-// chunk's loads are in flight during this chunk's matrix-core steps.  A's LDS
-// rows are padded to 17 elements so that the fragment reads (16 rows at one k)
-// spread over the banks.
+// so sixteen consecutive lanes read sixteen consecutive elements.  Double
+// buffered: the next chunk's raw h (and s) values are loaded into registers
+// before this chunk's matrix-core steps and turned into operand values (h - z s,
+// the conjugation, zeros outside the matrix) only when they are stored to LDS
+// after those steps, so no arithmetic waits on the loads in between.  Indices
+// are clamped, so every load is in bounds; a chunk past K contributes zeros.
+// A's LDS rows are padded to 17 elements so that the fragment reads (16 rows at
+// one k) spread over the banks.  Matrix-core work is skipped, with wave-uniform
+// control flow, for 16 x 16 fragment tiles outside the output and for k-steps
+// past K, so edge tiles cost what they cover at 16-element granularity.
 template <bool SA, bool SB>
 __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
   typedef mfma<double>::v4 v4;
@@ -65,6 +63,10 @@ __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
   const int i0 = ti * 64, j0 = tj * 64;
   const cx<double> zb = a.z ? a.z[b] : cx<double>{0, 0};
   const OpView A = op_view(a.A), Bv = op_view(a.B_);
+  const cx<double>* const Ah = A.h + (int64_t)b * A.bstride;
+  const cx<double>* const Bh = Bv.h + (int64_t)b * Bv.bstride;
+  const cx<double>* const Asp = SA ? A.s + (int64_t)b * A.sstride : nullptr;
+  const cx<double>* const Bsp = SB ? Bv.s + (int64_t)b * Bv.sstride : nullptr;
   const int t = threadIdx.x;
   const int lane = t & (WAVE - 1), w = t / WAVE;
   const int li = lane & 15, lk = lane >> 4;
@@ -72,36 +74,38 @@ __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
   // load assignment
   const int ak = t & 15, ar = t >> 4;  // A: column ak of rows ar + 16 u
   const int bk = t >> 4, bc = t & 15;  // B: row bk, columns bc + 16 u
-  int arow[4], bcol[4];
+  int64_t aoff[4], boff[4];
   bool arok[4], bcok[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int r = i0 + ar + 16 * u, c = j0 + bc + 16 * u;
     arok[u] = r < a.m;
     bcok[u] = c < a.n;
-    arow[u] = arok[u] ? r : a.m - 1;
-    bcol[u] = bcok[u] ? c : a.n - 1;
+    aoff[u] = (int64_t)(arok[u] ? r : a.m - 1) * A.rs;
+    boff[u] = (int64_t)(bcok[u] ? c : a.n - 1) * Bv.cs;
   }
-  cx<double> ra[4], rb[4];
+  cx<double> ra[4], rb[4], rsa[4], rsb[4];
   auto load_chunk = [&](int k0) __attribute__((always_inline)) {
-    const int ka = k0 + ak, kb = k0 + bk;
-    const bool kaok = ka < K, kbok = kb < K;
-    const int kac = kaok ? ka : K - 1, kbc = kbok ? kb : K - 1;
+    const int64_t ka = (int64_t)min(k0 + ak, K - 1) * A.cs;
+    const int64_t kb = (int64_t)min(k0 + bk, K - 1) * Bv.rs;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const cx<double> va = view_at<SA>(A, b, arow[u], kac, zb);
-      const bool oa = arok[u] && kaok;
-      ra[u] = {oa ? va.re : 0.0, oa ? va.im : 0.0};
-      const cx<double> vb = view_at<SB>(Bv, b, kbc, bcol[u], zb);
-      const bool ob = bcok[u] && kbok;
-      rb[u] = {ob ? vb.re : 0.0, ob ? vb.im : 0.0};
+      ra[u] = Ah[aoff[u] + ka];
+      if (SA) rsa[u] = Asp[aoff[u] + ka];
+      rb[u] = Bh[boff[u] + kb];
+      if (SB) rsb[u] = Bsp[boff[u] + kb];
     }
   };
-  auto store_chunk = [&](int buf) __attribute__((always_inline)) {
+  auto store_chunk = [&](int k0, int buf) __attribute__((always_inline)) {
+    const bool kaok = k0 + ak < K, kbok = k0 + bk < K;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      As[buf][(ar + 16 * u) * LDAS + ak] = ra[u];
-      Bs[buf][bk * LDBS + bc + 16 * u] = rb[u];
+      cx<double> va = ra[u], vb = rb[u];
+      if (SA) va = va - zb * rsa[u];
+      if (SB) vb = vb - zb * rsb[u];
+      const bool oa = arok[u] && kaok, ob = bcok[u] && kbok;
+      As[buf][(ar + 16 * u) * LDAS + ak] = {oa ? va.re : 0.0, oa ? A.csign * va.im : 0.0};
+      Bs[buf][bk * LDBS + bc + 16 * u] = {ob ? vb.re : 0.0, ob ? Bv.csign * vb.im : 0.0};
     }
   };
   v4 cr[2][2], ci[2][2];
@@ -110,35 +114,69 @@ __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
 #pragma unroll
     for (int y = 0; y < 2; ++y) cr[x][y] = ci[x][y] = v4{0, 0, 0, 0};
   const int rbase = 32 * (w / 2), cbase = 32 * (w % 2);
-  const int nchunks = (K + KC - 1) / KC;
-  load_chunk(0);
-  store_chunk(0);
-  __syncthreads();
-  for (int ch = 0; ch < nchunks; ++ch) {
-    const int buf = ch & 1;
-    const bool more = ch + 1 < nchunks;
-    if (more) load_chunk((ch + 1) * KC);
+  // NK matrix-core k-steps of the chunk in LDS buffer buf, for the RX x CY
+  // fragment tiles of this wave that overlap the output (the others would only
+  // multiply the zero padding).
+  auto ksteps = [&](auto RXc, auto CYc, auto NKc, int buf) __attribute__((always_inline)) {
+    constexpr int RX = decltype(RXc)::value, CY = decltype(CYc)::value;
+    constexpr int NK = decltype(NKc)::value;
 #pragma unroll
-    for (int kk = 0; kk < KC / 4; ++kk) {
+    for (int kk = 0; kk < NK; ++kk) {
       const int kl = 4 * kk + lk;
       cx<double> pa[2], pb[2];
 #pragma unroll
-      for (int x = 0; x < 2; ++x) pa[x] = As[buf][(rbase + 16 * x + li) * LDAS + kl];
+      for (int x = 0; x < RX; ++x) pa[x] = As[buf][(rbase + 16 * x + li) * LDAS + kl];
 #pragma unroll
-      for (int y = 0; y < 2; ++y) pb[y] = Bs[buf][kl * LDBS + cbase + 16 * y + li];
+      for (int y = 0; y < CY; ++y) pb[y] = Bs[buf][kl * LDBS + cbase + 16 * y + li];
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+      for (int x = 0; x < RX; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) {
+        for (int y = 0; y < CY; ++y) {
           cr[x][y] = mfma<double>::op(pa[x].re, pb[y].re, cr[x][y]);
           cr[x][y] = mfma<double>::op(-pa[x].im, pb[y].im, cr[x][y]);
           ci[x][y] = mfma<double>::op(pa[x].re, pb[y].im, ci[x][y]);
           ci[x][y] = mfma<double>::op(pa[x].im, pb[y].re, ci[x][y]);
         }
     }
-    if (more) store_chunk(buf ^ 1);
-    __syncthreads();
-  }
+  };
+  // K = nfull KC + rem.  The loop body has no branches, so the accumulators
+  // stay in one register class across iterations; its last iteration loads the
+  // chunk after nfull - 1 (the remainder, or a chunk past K: clamped in bounds,
+  // zeroed on store) into the other buffer, which the remainder steps then read.
+  const int nfull = K / KC, rem = K % KC;
+  auto run = [&](auto RXc, auto CYc) __attribute__((always_inline)) {
+    for (int ch = 0; ch < nfull; ++ch) {
+      const int buf = ch & 1;
+      load_chunk((ch + 1) * KC);
+      ksteps(RXc, CYc, std::integral_constant<int, KC / 4>(), buf);
+      store_chunk((ch + 1) * KC, buf ^ 1);
+      __syncthreads();
+    }
+    const int buf = nfull & 1;
+    switch ((rem + 3) / 4) {
+      case 1: ksteps(RXc, CYc, std::integral_constant<int, 1>(), buf); break;
+      case 2: ksteps(RXc, CYc, std::integral_constant<int, 2>(), buf); break;
+      case 3: ksteps(RXc, CYc, std::integral_constant<int, 3>(), buf); break;
+      case 4: ksteps(RXc, CYc, std::integral_constant<int, 4>(), buf); break;
+      default: break;
+    }
+  };
+  load_chunk(0);
+  store_chunk(0, 0);
+  __syncthreads();
+  // fragment tiles of this wave inside the output, wave-uniform
+  const int wr = __builtin_amdgcn_readfirstlane(i0 + rbase);
+  const int wc = __builtin_amdgcn_readfirstlane(j0 + cbase);
+  const int RX = wr + 16 < a.m ? 2 : (wr < a.m ? 1 : 0);
+  const int CY = wc + 16 < a.n ? 2 : (wc < a.n ? 1 : 0);
+  typedef std::integral_constant<int, 0> I0;
+  typedef std::integral_constant<int, 1> I1;
+  typedef std::integral_constant<int, 2> I2;
+  if (RX == 2 && CY == 2) run(I2(), I2());
+  else if (RX == 2 && CY == 1) run(I2(), I1());
+  else if (RX == 1 && CY == 2) run(I1(), I2());
+  else if (RX == 1 && CY == 1) run(I1(), I1());
+  else run(I0(), I0());  // loads and barriers only
   cx<double>* C = a.C + (int64_t)b * a.cstride;
   const cx<double> alpha = a.alpha, beta = a.beta;
   const bool use_beta = (beta.re != 0.0 || beta.im != 0.0);
