@@ -49,6 +49,7 @@ _SIGS = {
     "dpnegf_zgemm_batched": ([_vp, _i32, _i32, _i32, _i32, _vp, _i32, _i64, _i32, _vp, _i32, _i64,
                               _i32, _f64, _f64, _f64, _f64, _vp, _i32, _i64], _i32),
     "dpnegf_diag_product": ([_vp, _vp, _vp, _i64, _vp, _i64, _i32, _i32, _i32, _i32], _i32),
+    "dpnegf_zinv_batched": ([_vp, _vp, _i32, _i32, _vp], _i32),
     "dpnegf_rgf_transmission_blocked_workspace_elems": ([_i32, _vp, _i32], _i64),
     "dpnegf_autotune_query": ([_i32, _vp, _i64], _i64),
     "dpnegf_rgf_transmission_blocked": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _i32,

@@ -250,6 +250,17 @@ int dpnegf_zgemm_batched(void* stream, int B, int m, int n, int k, const void* A
   return 0;
 }
 
+int dpnegf_zinv_batched(void* stream, void* A, int n, int B, int32_t* status) {
+  if (n < 1 || n > negf::GJ_NMAX_LARGE)
+    return fail(DPNEGF_ERR_ARG, "n=%d outside 1..%d", n, negf::GJ_NMAX_LARGE);
+  if (B < 0) return fail(DPNEGF_ERR_ARG, "negative batch");
+  if (B == 0) return 0;
+  if (!A) return fail(DPNEGF_ERR_ARG, "null device pointer");
+  hipError_t e = negf::zinv_batched((hipStream_t)stream, (cx<double>*)A, n, B, status);
+  if (e != hipSuccess) return hip_fail(e, "zinv_batched");
+  return 0;
+}
+
 int dpnegf_diag_product(void* stream, void* out, const void* A, int64_t strideA, const void* Bm,
                         int64_t strideB, int m, int n, int B, int accumulate) {
   if (B < 0 || m < 0 || n < 0) return fail(DPNEGF_ERR_ARG, "negative size");

@@ -622,6 +622,12 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
   return hipGetLastError();
 }
 
+hipError_t zinv_batched(hipStream_t st, cx<double>* A, int n, int B, int32_t* status) {
+  if (n < 1 || n > GJ_NMAX_LARGE || B < 0) return hipErrorInvalidValue;
+  if (B == 0) return hipSuccess;
+  return inverse(st, A, n, B, status, nullptr);
+}
+
 static Opnd mat(const cx<double>* h, int ld, int64_t bstride, int op = OP_N,
                 const cx<double>* s = nullptr, int64_t sstride = 0);
 static Opnd mat(const cx<double>* h, int ld, int64_t bstride, int op, const cx<double>* s,

@@ -12,6 +12,8 @@ constexpr int GJ_NMAX_LARGE = 128;
 constexpr int BLOCK_NMAX = 256;
 // Workspace of the batched inverse for n x n blocks (0 up to GJ_NMAX_LARGE).
 int64_t inverse_workspace(int n, int B);
+// In-place inverse of B contiguous n x n blocks, 1 <= n <= GJ_NMAX_LARGE.
+hipError_t zinv_batched(hipStream_t st, cx<double>* A, int n, int B, int32_t* status);
 
 // Operand value (i, k) = op(h[b] - z_b s[b]) with s optional, row-major ld.
 struct Opnd {

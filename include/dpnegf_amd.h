@@ -171,6 +171,13 @@ int dpnegf_zgemm_batched(void* stream, int B, int m, int n, int k, const void* A
                          double alpha_re, double alpha_im, double beta_re, double beta_im, void* C,
                          int ldc, int64_t strideC);
 
+/* In place: A[b] <- A[b]^-1 for B contiguous row-major n x n complex128
+ * blocks, 1 <= n <= 128 (Gauss-Jordan with partial pivoting, one workgroup
+ * per block).  status[b] (optional, [B]) gets bit 1 or-ed in for an exactly
+ * singular pivot.  The block inverse of the reference's RGF sweep,
+ * tLA.solve(-M, I) (dpnegf/negf/recursive_green_cal.py:89-93, 199-206). */
+int dpnegf_zinv_batched(void* stream, void* A, int n, int B, int32_t* status);
+
 /* out[b][i] (+)= sum_j A[b][i][j] B[b][j][i]: the diagonal of A B (A m x n,
  * B n x m, contiguous rows), for traces and per-orbital spectra
  * (dpnegf/negf/device_property.py:404, 438, 461). */

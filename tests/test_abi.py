@@ -44,6 +44,11 @@ def test_argument_errors_are_reported_without_a_device():
     with pytest.raises(ValueError):
         nat.call("dpnegf_rgf_full", None, 0, None, None, 1, None, None, 0, None, 0, 1,
                  *([None] * 17))
+    for n in (0, 129):
+        assert lib.dpnegf_zinv_batched(None, None, n, 1, None) < 0
+        assert b"outside" in lib.dpnegf_last_error()
+    assert lib.dpnegf_zinv_batched(None, None, 16, 1, None) < 0  # null pointer
+    assert lib.dpnegf_zinv_batched(None, None, 16, 0, None) == 0  # empty batch
 
 
 @pytest.mark.skipif(not os.path.exists(nat.lib_path()), reason="library not built")
