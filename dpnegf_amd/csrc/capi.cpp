@@ -197,28 +197,60 @@ int64_t dpnegf_rgf_grad_workspace_elems(int K, const int32_t* h_bsz, int B) {
   return negf::rgf_grad_workspace(K, h_bsz, B);
 }
 
-int dpnegf_rgf_transmission_grad(void* stream, int K, const int32_t* h_bsz, const void* arena,
-                                 int B, const void* z, const void* sigL, int nL, const void* sigR,
-                                 int nR, int ortho, const double* w, void* grad_hd, void* grad_hu,
-                                 void* grad_hl, void* work, int32_t* status) {
+// Arguments of both adjoint entries; 1 when there is nothing to do (B == 0),
+// < 0 on an argument error, 0 otherwise.
+static int grad_args(negf::RgfGradArgs& a, int K, const int32_t* h_bsz, const void* arena, int B,
+                     const void* z, const void* sigL, int nL, const void* sigR, int nR, int ortho,
+                     const double* w, void* grad_hd, void* grad_hu, void* grad_hl, void* work,
+                     int32_t* status) {
   int nmax = 0;
   if (int r = check_blocks(K, h_bsz, DPNEGF_FULL_NMAX, &nmax)) return r;
   if (B < 0) return fail(DPNEGF_ERR_ARG, "B must be >= 0");
-  if (B == 0) return 0;
+  if (B == 0) return 1;
   if (B > 65535) return fail(DPNEGF_ERR_ARG, "B=%d exceeds 65535 per call", B);
   if (!arena || !z || !sigL || !sigR || !w || !grad_hd || !work)
     return fail(DPNEGF_ERR_ARG, "null device pointer");
   if (K > 1 && (!grad_hu || !grad_hl)) return fail(DPNEGF_ERR_ARG, "grad_hu/grad_hl required");
   if (nL < 1 || nL > h_bsz[0] || nR < 1 || nR > h_bsz[K - 1])
     return fail(DPNEGF_ERR_ARG, "self-energy larger than its corner block");
-  negf::RgfGradArgs a;
   a.K = K; a.B = B; a.ortho = ortho; a.bsz = h_bsz; a.arena = (const cx<double>*)arena;
   a.z = (const cx<double>*)z; a.sigL = (const cx<double>*)sigL; a.nL = nL;
   a.sigR = (const cx<double>*)sigR; a.nR = nR; a.w = w; a.grad_hd = (cx<double>*)grad_hd;
   a.grad_hu = (cx<double>*)grad_hu; a.grad_hl = (cx<double>*)grad_hl; a.work = (cx<double>*)work;
   a.status = status;
+  return 0;
+}
+
+int dpnegf_rgf_transmission_grad(void* stream, int K, const int32_t* h_bsz, const void* arena,
+                                 int B, const void* z, const void* sigL, int nL, const void* sigR,
+                                 int nR, int ortho, const double* w, void* grad_hd, void* grad_hu,
+                                 void* grad_hl, void* work, int32_t* status) {
+  negf::RgfGradArgs a;
+  const int r = grad_args(a, K, h_bsz, arena, B, z, sigL, nL, sigR, nR, ortho, w, grad_hd, grad_hu,
+                          grad_hl, work, status);
+  if (r) return r < 0 ? r : 0;
   hipError_t e = negf::rgf_grad(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "rgf_transmission_grad");
+  return 0;
+}
+
+int64_t dpnegf_rgf_grad_ckpt_workspace_elems(int K, const int32_t* h_bsz, int B, int stride) {
+  if (!h_bsz || K < 1 || B < 0 || stride < 1) return -1;
+  return negf::rgf_grad_ckpt_workspace(K, h_bsz, B, stride);
+}
+
+int dpnegf_rgf_transmission_grad_ckpt(void* stream, int K, const int32_t* h_bsz,
+                                      const void* arena, int B, const void* z, const void* sigL,
+                                      int nL, const void* sigR, int nR, int ortho, const double* w,
+                                      void* grad_hd, void* grad_hu, void* grad_hl, int stride,
+                                      void* work, int32_t* status) {
+  if (stride < 1) return fail(DPNEGF_ERR_ARG, "checkpoint stride %d < 1", stride);
+  negf::RgfGradArgs a;
+  const int r = grad_args(a, K, h_bsz, arena, B, z, sigL, nL, sigR, nR, ortho, w, grad_hd, grad_hu,
+                          grad_hl, work, status);
+  if (r) return r < 0 ? r : 0;
+  hipError_t e = negf::rgf_grad_ckpt(a, stride, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "rgf_transmission_grad_ckpt");
   return 0;
 }
 

@@ -1145,4 +1145,226 @@ hipError_t rgf_grad(const RgfGradArgs& g, hipStream_t st) {
   return hipSuccess;
 }
 
+// ---- adjoint with sqrt(K) checkpointing ------------------------------------
+// The same gradients as rgf_grad, in O(K / s + s) blocks per energy instead of
+// O(K): the forward sweep keeps the left-connected g_c and the first-row
+// product R_c = g_0 U_0 ... g_{c-1} U_{c-1} only at c = 0, s, 2s, ...; the
+// gradient sweep then runs down the blocks one segment [c, c + s) at a time,
+// from the right, recomputing the segment's g_q and R_q from its checkpoint
+// and carrying G_{q+1,q+1}, G_{q+1,K-1} and D_{q+1} = C G_{0,q+1} across
+// segment boundaries.  Cost: one more forward sweep than rgf_grad.
+
+// Per-energy element counts, in workspace order.
+struct CkptLayout {
+  int nseg, nmax;
+  int64_t ck, m2, per;
+};
+static CkptLayout ckpt_layout(int K, const int32_t* bsz, int stride) {
+  CkptLayout L;
+  L.nmax = 0;
+  for (int q = 0; q < K; ++q) L.nmax = std::max(L.nmax, (int)bsz[q]);
+  L.nseg = (K + stride - 1) / stride;
+  L.ck = 0;
+  for (int j = 0; j < L.nseg; ++j) {
+    const int64_t nc = bsz[j * stride];
+    L.ck += nc * nc + (j > 0 ? (int64_t)bsz[0] * nc : 0);
+  }
+  L.m2 = (int64_t)L.nmax * L.nmax;
+  // checkpoints | segment g, R (stride each) | carries G_qq, G_{q,K-1}, D (x2) |
+  // C, Gamma_L, Gamma_R | forward g, R (x2) | five temporaries
+  L.per = L.ck + (2 * (int64_t)stride + 6 + 3 + 4 + 5) * L.m2;
+  return L;
+}
+
+int64_t rgf_grad_ckpt_workspace(int K, const int32_t* bsz, int B, int stride) {
+  if (stride < 1) return -1;
+  const CkptLayout L = ckpt_layout(K, bsz, stride);
+  return (int64_t)B * L.per + inverse_workspace(L.nmax, B);
+}
+
+hipError_t rgf_grad_ckpt(const RgfGradArgs& g, int stride, hipStream_t st) {
+  const int K = g.K, B = g.B;
+  if (stride < 1) return hipErrorInvalidValue;
+  const std::vector<int> n(g.bsz, g.bsz + K);
+  const CkptLayout Ly = ckpt_layout(K, g.bsz, stride);
+  const int n0 = n[0], nK = n[K - 1], nseg = Ly.nseg;
+  const int64_t M2 = (int64_t)B * Ly.m2;
+  std::vector<int64_t> hoff(K), gsq(K), gx(K > 1 ? K - 1 : 1);
+  int64_t sh = 0, s_sq = 0, s_x = 0;
+  for (int q = 0; q < K; ++q) {
+    hoff[q] = sh; sh += 2 * (int64_t)n[q] * n[q];
+    gsq[q] = s_sq; s_sq += (int64_t)n[q] * n[q];
+    if (q + 1 < K) {
+      gx[q] = s_x; s_x += (int64_t)n[q] * n[q + 1];
+      sh += 4 * (int64_t)n[q] * n[q + 1];
+    }
+  }
+  const cx<double>* H = g.arena;
+  auto hd = [&](int q) { return H + hoff[q]; };
+  auto sdp = [&](int q) { return H + hoff[q] + (int64_t)n[q] * n[q]; };
+  auto hu = [&](int q) { return H + hoff[q] + 2 * (int64_t)n[q] * n[q]; };
+  auto suu = [&](int q) { return hu(q) + (int64_t)n[q] * n[q + 1]; };
+  auto hl = [&](int q) { return suu(q) + (int64_t)n[q] * n[q + 1]; };
+  auto sl = [&](int q) { return hl(q) + (int64_t)n[q + 1] * n[q]; };
+  auto Uop = [&](int q) { return mat(hu(q), n[q + 1], 0, OP_N, g.ortho ? nullptr : suu(q), 0); };
+  auto Lop = [&](int q) { return mat(hl(q), n[q], 0, OP_N, g.ortho ? nullptr : sl(q), 0); };
+  auto M = [&](const cx<double>* p, int ld, int rows, int op = OP_N) {
+    return mat(p, ld, (int64_t)rows * ld, op);
+  };
+  // workspace, in the order of ckpt_layout
+  cx<double>* w = g.work;
+  std::vector<cx<double>*> ckg(nseg), ckR(nseg, nullptr);
+  for (int j = 0; j < nseg; ++j) {
+    const int nc = n[j * stride];
+    ckg[j] = w; w += (int64_t)B * nc * nc;
+    if (j > 0) { ckR[j] = w; w += (int64_t)B * n0 * nc; }
+  }
+  cx<double>* segg = w; w += stride * M2;  // slot i: g_{c+i}
+  cx<double>* segR = w; w += stride * M2;  // slot i: R_{c+i} (n0 x n_{c+i}); unused for q = 0
+  cx<double>* Gd[2] = {w, w + M2}; w += 2 * M2;  // G_qq
+  cx<double>* Gc[2] = {w, w + M2}; w += 2 * M2;  // G_{q,K-1}
+  cx<double>* D[2] = {w, w + M2}; w += 2 * M2;   // C G_{0,q}
+  cx<double>* C = w; w += M2;
+  cx<double>* gamL = w; w += M2;
+  cx<double>* gamR = w; w += M2;
+  cx<double>* fg[2] = {w, w + M2}; w += 2 * M2;
+  cx<double>* fR[2] = {w, w + M2}; w += 2 * M2;
+  cx<double>* t0 = w; w += M2;
+  cx<double>* t1 = w; w += M2;
+  cx<double>* t2 = w; w += M2;
+  cx<double>* t3 = w; w += M2;
+  cx<double>* Wm = w; w += M2;
+  cx<double>* inv_ws = w;
+  const cx<double> one{1, 0}, mone{-1, 0}, zero{0, 0};
+  auto slot = [&](cx<double>* base, int i) { return base + i * M2; };
+  auto copy = [&](cx<double>* dst, const cx<double>* src, int64_t elems) {
+    return hipMemcpyAsync(dst, src, sizeof(cx<double>) * elems, hipMemcpyDeviceToDevice, st);
+  };
+  auto build = [&](cx<double>* dst, int q) {
+    const int64_t tot = (int64_t)B * n[q] * n[q];
+    const int grid = (int)std::min<int64_t>((tot + 255) / 256, 65535);
+    hipLaunchKernelGGL(bbuild_kernel, dim3(grid), dim3(256), 0, st, dst, hd(q), sdp(q), n[q], g.z, B,
+                       q == 0 ? g.sigL : nullptr, g.nL, q == K - 1 ? g.sigR : nullptr, g.nR);
+    return hipGetLastError();
+  };
+  // g_{q+1} from g_q  (:198-207)
+  auto forward = [&](const cx<double>* gq, int q, cx<double>* gnext) -> hipError_t {
+    const int nq = n[q], n1 = n[q + 1];
+    CHK(gemm(st, B, nq, n1, nq, M(gq, nq, nq), Uop(q), one, zero, t0, (int64_t)nq * n1, n1, g.z));
+    CHK(build(gnext, q + 1));
+    CHK(gemm(st, B, n1, n1, nq, Lop(q), M(t0, n1, nq), mone, one, gnext, (int64_t)n1 * n1, n1, g.z));
+    return inverse(st, gnext, n1, B, g.status, inv_ws);
+  };
+  // R_{q+1} = R_q g_q U_q, R_1 = g_0 U_0  (Rq unused for q = 0)
+  auto rnext = [&](const cx<double>* Rq, const cx<double>* gq, int q, cx<double>* Rout) -> hipError_t {
+    const int nq = n[q], n1 = n[q + 1];
+    const cx<double>* left = gq;
+    if (q > 0) {
+      CHK(gemm(st, B, n0, nq, nq, M(Rq, nq, n0), M(gq, nq, nq), one, zero, t1, (int64_t)n0 * nq, nq,
+               g.z));
+      left = t1;
+    }
+    return gemm(st, B, n0, n1, nq, M(left, nq, q > 0 ? n0 : nq), Uop(q), one, zero, Rout,
+                (int64_t)n0 * n1, n1, g.z);
+  };
+  auto reduce = [&](cx<double>* out, const cx<double>* in, int m_, int n_) {
+    hipLaunchKernelGGL(breduce_conjt_kernel, dim3((m_ * n_ + 255) / 256), dim3(256), 0, st, out, in,
+                       m_, n_, B);
+    return hipGetLastError();
+  };
+
+  // ---- forward: checkpoints and G_{0,K-1} = R_{K-1} g_{K-1}
+  CHK(build(fg[0], 0));
+  CHK(inverse(st, fg[0], n0, B, g.status, inv_ws));
+  CHK(copy(ckg[0], fg[0], (int64_t)B * n0 * n0));
+  int cg = 0, cr = 0;
+  for (int q = 0; q + 1 < K; ++q) {
+    CHK(rnext(fR[cr], fg[cg], q, fR[1 - cr]));
+    CHK(forward(fg[cg], q, fg[1 - cg]));
+    cg = 1 - cg;
+    cr = 1 - cr;
+    if ((q + 1) % stride == 0) {
+      const int j = (q + 1) / stride, nc = n[q + 1];
+      CHK(copy(ckg[j], fg[cg], (int64_t)B * nc * nc));
+      CHK(copy(ckR[j], fR[cr], (int64_t)B * n0 * nc));
+    }
+  }
+  const cx<double>* g0K = fg[cg];  // K = 1: G_{0,0} = g_0
+  if (K > 1) {
+    CHK(gemm(st, B, n0, nK, nK, M(fR[cr], nK, n0), M(fg[cg], nK, nK), one, zero, t2,
+             (int64_t)n0 * nK, nK, g.z));
+    g0K = t2;
+  }
+  // C = Gamma_R G_{0,K-1}^H (2 w Gamma_L)      [nK x n0]
+  auto grid_of = [](int64_t tot) { return (int)std::min<int64_t>((tot + 255) / 256, 65535); };
+  hipLaunchKernelGGL(bgamma_pad_kernel, dim3(grid_of((int64_t)B * n0 * n0)), dim3(256), 0, st, gamL,
+                     g.sigL, g.nL, n0, B, 0, g.w, 2.0);
+  hipLaunchKernelGGL(bgamma_pad_kernel, dim3(grid_of((int64_t)B * nK * nK)), dim3(256), 0, st, gamR,
+                     g.sigR, g.nR, nK, B, 1, (const double*)nullptr, 1.0);
+  CHK(hipGetLastError());
+  CHK(gemm(st, B, nK, n0, nK, M(gamR, nK, nK), M(g0K, nK, n0, OP_H), one, zero, Wm,
+           (int64_t)nK * n0, n0, g.z));
+  CHK(gemm(st, B, nK, n0, n0, M(Wm, n0, nK), M(gamL, n0, n0), one, zero, C, (int64_t)nK * n0, n0,
+           g.z));
+
+  // ---- gradient sweep, segment by segment from the right
+  int cur = 0;  // carries of block q in [cur], of block q + 1 in [1 - cur]
+  for (int j = nseg - 1; j >= 0; --j) {
+    const int c = j * stride, e = std::min(c + stride, K);
+    CHK(copy(slot(segg, 0), ckg[j], (int64_t)B * n[c] * n[c]));
+    if (c > 0) CHK(copy(slot(segR, 0), ckR[j], (int64_t)B * n0 * n[c]));
+    for (int i = 0; c + i + 1 < e; ++i) {
+      CHK(forward(slot(segg, i), c + i, slot(segg, i + 1)));
+      CHK(rnext(slot(segR, i), slot(segg, i), c + i, slot(segR, i + 1)));
+    }
+    for (int q = e - 1; q >= c; --q) {
+      const int i = q - c, nq = n[q];
+      const cx<double>* gq = slot(segg, i);
+      if (q == K - 1) {
+        CHK(copy(Gd[cur], gq, (int64_t)B * nq * nq));
+        CHK(copy(Gc[cur], gq, (int64_t)B * nq * nK));
+      } else {
+        const int n1 = n[q + 1];
+        // gU = g_q U_q;  G_{q+1,q} = G_{q+1,q+1} L_q g_q;  G_qq = g_q + gU G_{q+1,q}
+        CHK(gemm(st, B, nq, n1, nq, M(gq, nq, nq), Uop(q), one, zero, t0, (int64_t)nq * n1, n1, g.z));
+        CHK(gemm(st, B, n1, nq, n1, M(Gd[1 - cur], n1, n1), Lop(q), one, zero, t1, (int64_t)n1 * nq,
+                 nq, g.z));
+        CHK(gemm(st, B, n1, nq, nq, M(t1, nq, n1), M(gq, nq, nq), one, zero, t2, (int64_t)n1 * nq, nq,
+                 g.z));
+        CHK(copy(Gd[cur], gq, (int64_t)B * nq * nq));
+        CHK(gemm(st, B, nq, nq, n1, M(t0, n1, nq), M(t2, nq, n1), one, one, Gd[cur], (int64_t)nq * nq,
+                 nq, g.z));
+        // G_{q,K-1} = gU G_{q+1,K-1}
+        CHK(gemm(st, B, nq, nK, n1, M(t0, n1, nq), M(Gc[1 - cur], nK, n1), one, zero, Gc[cur],
+                 (int64_t)nq * nK, nK, g.z));
+      }
+      // G_{0,q} = R_q G_qq (G_{0,0} = G_00);  D_q = C G_{0,q}
+      const cx<double>* g0q = Gd[cur];
+      if (q > 0) {
+        CHK(gemm(st, B, n0, nq, nq, M(slot(segR, i), nq, n0), M(Gd[cur], nq, nq), one, zero, t3,
+                 (int64_t)n0 * nq, nq, g.z));
+        g0q = t3;
+      }
+      CHK(gemm(st, B, nK, nq, n0, M(C, n0, nK), M(g0q, nq, n0), one, zero, D[cur], (int64_t)nK * nq,
+               nq, g.z));
+      // grad_hd[q] = sum_b (G_{q,K-1} D_q)^H
+      CHK(gemm(st, B, nq, nq, nK, M(Gc[cur], nK, nq), M(D[cur], nq, nK), one, zero, Wm,
+               (int64_t)nq * nq, nq, g.z));
+      CHK(reduce(g.grad_hd + gsq[q], Wm, nq, nq));
+      if (q + 1 < K) {
+        const int n1 = n[q + 1];
+        // grad_hu[q] = sum_b (G_{q+1,K-1} D_q)^H ;  grad_hl[q] = sum_b (G_{q,K-1} D_{q+1})^H
+        CHK(gemm(st, B, n1, nq, nK, M(Gc[1 - cur], nK, n1), M(D[cur], nq, nK), one, zero, Wm,
+                 (int64_t)n1 * nq, nq, g.z));
+        CHK(reduce(g.grad_hu + gx[q], Wm, n1, nq));
+        CHK(gemm(st, B, nq, n1, nK, M(Gc[cur], nK, nq), M(D[1 - cur], n1, nK), one, zero, Wm,
+                 (int64_t)nq * n1, n1, g.z));
+        CHK(reduce(g.grad_hl + gx[q], Wm, nq, n1));
+      }
+      cur = 1 - cur;
+    }
+  }
+  return hipSuccess;
+}
+
 }  // namespace negf

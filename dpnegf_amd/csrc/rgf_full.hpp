@@ -74,6 +74,11 @@ struct RgfGradArgs {
   int32_t* status;
 };
 
+// Adjoint with sqrt(K)-style checkpointing every `stride` blocks (same outputs
+// as rgf_grad; work: rgf_grad_ckpt_workspace elements, -1 for stride < 1).
+int64_t rgf_grad_ckpt_workspace(int K, const int32_t* bsz, int B, int stride);
+hipError_t rgf_grad_ckpt(const RgfGradArgs& g, int stride, hipStream_t st);
+
 hipError_t diag_product(cx<double>* out, const cx<double>* A, int64_t sa, const cx<double>* Bm,
                         int64_t sb, int m, int n, int B, int accumulate, hipStream_t st);
 hipError_t zgemm_batched(int B, int m, int n, int k, const cx<double>* A, int lda, int64_t sa,

@@ -8,7 +8,16 @@ complex-gradient convention, dJ/dRe + i dJ/dIm).  The reference has no such
 kernel; it relies on autograd through ``recursive_gf``
 (``dpnegf/negf/recursive_green_cal.py:89-112``), which is what the tests
 compare against, together with central finite differences.
+
+Memory: the plain adjoint keeps every block of the sweep for all B energies,
+O(K) blocks per energy.  With ``checkpoint`` it keeps the left-connected g_q
+and the first-row products only every s blocks and recomputes each segment of
+s blocks during the gradient sweep (``dpnegf_rgf_transmission_grad_ckpt``):
+O(K / s + s) blocks per energy for one more forward sweep, s = ceil(sqrt(K))
+by default.
 """
+import math
+
 import torch
 
 from .. import _native as nat
@@ -19,7 +28,7 @@ from .recursive_green_cal import BTDSystems, rgf_transmission
 class _Transmission(torch.autograd.Function):
     @staticmethod
     def forward(ctx, meta, z, sigL, sigR, *blocks):
-        K, sd, su, sl, precision = meta
+        K, sd, su, sl, precision, checkpoint = meta
         hd = list(blocks[:K])
         hu = list(blocks[K:2 * K - 1])
         hl = list(blocks[2 * K - 1:3 * K - 2])
@@ -28,6 +37,7 @@ class _Transmission(torch.autograd.Function):
         T = rgf_transmission(bs, z, sigL, sigR, precision=precision)
         ctx.K = K
         ctx.bs = bs
+        ctx.checkpoint = checkpoint
         ctx.save_for_backward(z, sigL, sigR)
         return T
 
@@ -41,7 +51,11 @@ class _Transmission(torch.autograd.Function):
         B = z.shape[0]
         w = gT.detach().to(device=dev, dtype=torch.float64).contiguous()
         h_bsz = nat.host_i32(sizes)
-        nwork = nat.load().dpnegf_rgf_grad_workspace_elems(K, h_bsz, B)
+        stride = checkpoint_stride(ctx.checkpoint, sizes, B, dev)
+        if stride:
+            nwork = nat.load().dpnegf_rgf_grad_ckpt_workspace_elems(K, h_bsz, B, stride)
+        else:
+            nwork = nat.load().dpnegf_rgf_grad_workspace_elems(K, h_bsz, B)
         work = torch.empty(nwork, dtype=C128, device=dev)
         nsq = sum(n * n for n in sizes)
         nx = sum(sizes[q] * sizes[q + 1] for q in range(K - 1))
@@ -49,11 +63,14 @@ class _Transmission(torch.autograd.Function):
         ghu = torch.empty(max(nx, 1), dtype=C128, device=dev)
         ghl = torch.empty(max(nx, 1), dtype=C128, device=dev)
         status = torch.zeros(B, dtype=torch.int32, device=dev)
-        nat.call("dpnegf_rgf_transmission_grad", nat.stream_ptr(dev), K, h_bsz,
-                 nat.ptr(bs.arena), B, nat.ptr(z), nat.ptr(sigL), sigL.shape[-1],
-                 nat.ptr(sigR), sigR.shape[-1], int(bs.ortho), nat.ptr(w), nat.ptr(ghd),
-                 nat.ptr(ghu), nat.ptr(ghl),
-                 nat.ptr(work), nat.ptr(status))
+        args = (nat.stream_ptr(dev), K, h_bsz, nat.ptr(bs.arena), B, nat.ptr(z), nat.ptr(sigL),
+                sigL.shape[-1], nat.ptr(sigR), sigR.shape[-1], int(bs.ortho), nat.ptr(w),
+                nat.ptr(ghd), nat.ptr(ghu), nat.ptr(ghl))
+        if stride:
+            nat.call("dpnegf_rgf_transmission_grad_ckpt", *args, stride, nat.ptr(work),
+                     nat.ptr(status))
+        else:
+            nat.call("dpnegf_rgf_transmission_grad", *args, nat.ptr(work), nat.ptr(status))
         g_hd = [t[0] for t in split_blocks(ghd, 1, [(n, n) for n in sizes])]
         up = [(sizes[q], sizes[q + 1]) for q in range(K - 1)]
         g_hu = [t[0] for t in split_blocks(ghu, 1, up)]
@@ -61,12 +78,34 @@ class _Transmission(torch.autograd.Function):
         return (None, None, None, None, *g_hd, *g_hu, *g_hl)
 
 
-def transmission(energy, hd, hu, hl, sd, su, sl, sigL, sigR, eta, E_ref=0.0, precision="fp64"):
+def checkpoint_stride(checkpoint, sizes, B, dev, mem_fraction=0.5):
+    """Checkpoint stride of the adjoint (0: keep every block).
+
+    ``checkpoint``: None picks 0 when the plain adjoint's work buffer fits in
+    ``mem_fraction`` of the free device memory, else ceil(sqrt(K)); "sqrt"
+    always ceil(sqrt(K)); an int >= 1 that stride; 0 keeps every block."""
+    K = len(sizes)
+    sqrt_k = max(1, math.isqrt(K - 1) + 1) if K > 1 else 1
+    if checkpoint is None:
+        full = nat.load().dpnegf_rgf_grad_workspace_elems(K, nat.host_i32(sizes), B) * 16
+        free, _ = torch.cuda.mem_get_info(dev)
+        return 0 if full <= mem_fraction * free else sqrt_k
+    if checkpoint == "sqrt":
+        return sqrt_k
+    stride = int(checkpoint)
+    if stride < 0:
+        raise ValueError(f"checkpoint stride must be >= 0, got {checkpoint}")
+    return stride
+
+
+def transmission(energy, hd, hu, hl, sd, su, sl, sigL, sigR, eta, E_ref=0.0, precision="fp64",
+                 checkpoint=None):
     """Differentiable T(E) [B] of one system.
 
     ``energy`` [B] relative to ``E_ref``; ``sigL`` [B, nL, nL] / ``sigR``
     [B, nR, nR] the lead self-energies trimmed to their corners (treated as
-    constants).  Gradients flow to ``hd``, ``hu``, ``hl``.
+    constants).  Gradients flow to ``hd``, ``hu``, ``hl``.  ``checkpoint``:
+    memory mode of the backward pass (``checkpoint_stride``).
     """
     dev = hd[0].device
     nat.require_device(hd[0])
@@ -74,6 +113,6 @@ def transmission(energy, hd, hu, hl, sd, su, sl, sigL, sigR, eta, E_ref=0.0, pre
     z = (torch.as_tensor(energy).to(device=dev, dtype=C128).reshape(-1) + E_ref + 1j * eta)
     z = z.contiguous()
     meta = (K, [s.detach() for s in sd], [s.detach() for s in su], [s.detach() for s in sl],
-            precision)
+            precision, checkpoint)
     return _Transmission.apply(meta, z, sigL.detach().contiguous(), sigR.detach().contiguous(),
                                *hd, *hu, *hl)

@@ -153,6 +153,26 @@ int dpnegf_rgf_transmission_grad(void* stream, int K, const int32_t* h_bsz, cons
                                  int nR, int ortho, const double* w, void* grad_hd, void* grad_hu,
                                  void* grad_hl, void* work, int32_t* status);
 
+/* Complex elements of the work buffer dpnegf_rgf_transmission_grad_ckpt needs
+ * for checkpoint stride `stride` (-1 for stride < 1): B times
+ * (sum over checkpoints c = 0, s, 2s, ... of n_c^2 + n_0 n_c, less n_0^2)
+ * + B (2 s + 18) nmax^2, plus the large-block inverse's workspace. */
+int64_t dpnegf_rgf_grad_ckpt_workspace_elems(int K, const int32_t* h_bsz, int B, int stride);
+
+/* dpnegf_rgf_transmission_grad with checkpointing: the forward sweep keeps
+ * the left-connected g_c and the first-row product g_0 U_0 ... g_{c-1} U_{c-1}
+ * only at c = 0, s, 2s, ... (s = stride) and the gradient sweep recomputes each
+ * segment of s blocks from its checkpoint, so the work buffer holds
+ * O(K / s + s) blocks per energy instead of O(K) (s near sqrt(K) minimises
+ * it), for one more forward sweep.  Same arguments and outputs otherwise; the
+ * checkpointing has no counterpart in the reference, whose gradients come from
+ * torch autograd through recursive_gf (dpnegf/negf/recursive_green_cal.py:89-112). */
+int dpnegf_rgf_transmission_grad_ckpt(void* stream, int K, const int32_t* h_bsz,
+                                      const void* arena, int B, const void* z, const void* sigL,
+                                      int nL, const void* sigR, int nR, int ortho, const double* w,
+                                      void* grad_hd, void* grad_hu, void* grad_hl, int stride,
+                                      void* work, int32_t* status);
+
 /* out[e] = (accumulate ? out[e] : 0) + sum_b Re(c[b] * in[b * count + e]):
  * the pole / quadrature sum of the Ozaki density, Ozaki.integrate
  * (dpnegf/negf/density.py:171-181, 189-195), over the B points of a batch.

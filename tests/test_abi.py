@@ -81,3 +81,40 @@ def test_every_package_module_imports():
     import dpnegf_amd
     for m in pkgutil.walk_packages(dpnegf_amd.__path__, "dpnegf_amd."):
         importlib.import_module(m.name)
+
+
+@pytest.mark.skipif(not os.path.exists(nat.lib_path()), reason="library not built")
+def test_checkpointed_adjoint_workspace():
+    """The checkpointed adjoint's work buffer follows its stated formula and is
+    several times smaller than the full-storage one at 64 blocks of 64."""
+    lib = nat.load()
+    for sizes in ([64] * 64, [6, 9, 7, 8, 6, 10, 7, 9, 8, 6, 7, 9], [200, 90, 256, 120]):
+        K, h = len(sizes), nat.host_i32(sizes)
+        B, nmax, n0 = 3, max(sizes), sizes[0]
+        inv = lib.dpnegf_rgf_grad_ckpt_workspace_elems(K, h, B, K) - B * (
+            sum(sizes[c] ** 2 + n0 * sizes[c] for c in range(0, K, K)) - n0 * n0
+            + (2 * K + 18) * nmax ** 2)
+        assert inv >= 0  # the large-block inverse's workspace, independent of the stride
+        for s in (1, 2, 3, 8, K):
+            ck = sum(sizes[c] ** 2 + n0 * sizes[c] for c in range(0, K, s)) - n0 * n0
+            want = B * (ck + (2 * s + 18) * nmax ** 2) + inv
+            assert lib.dpnegf_rgf_grad_ckpt_workspace_elems(K, h, B, s) == want, (sizes, s)
+    h = nat.host_i32([64] * 64)
+    full = lib.dpnegf_rgf_grad_workspace_elems(64, h, 16)
+    ck = lib.dpnegf_rgf_grad_ckpt_workspace_elems(64, h, 16, 8)
+    assert ck * 4 < full
+    assert lib.dpnegf_rgf_grad_ckpt_workspace_elems(64, h, 16, 0) == -1
+    rc = lib.dpnegf_rgf_transmission_grad_ckpt(None, 1, nat.host_i32([4]), None, 1, None, None, 1,
+                                               None, 1, 1, None, None, None, None, 0, None, None)
+    assert rc < 0 and b"stride" in lib.dpnegf_last_error()
+
+
+def test_checkpoint_stride_choices():
+    from dpnegf_amd.negf.adjoint import checkpoint_stride
+    assert checkpoint_stride("sqrt", [8] * 10, 2, None) == 4
+    assert checkpoint_stride("sqrt", [8] * 9, 2, None) == 3
+    assert checkpoint_stride("sqrt", [8], 2, None) == 1
+    assert checkpoint_stride(5, [8] * 10, 2, None) == 5
+    assert checkpoint_stride(0, [8] * 10, 2, None) == 0
+    with pytest.raises(ValueError):
+        checkpoint_stride(-1, [8] * 10, 2, None)

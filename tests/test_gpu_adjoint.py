@@ -100,3 +100,41 @@ def test_adjoint_matches_autograd_and_fd(case):
             fd = (vals[0] - vals[1]) / (2 * delta)
             an = got[p][i, j].real.item() if direction == 1.0 else got[p][i, j].imag.item()
             assert abs(fd - an) <= 1e-5 * max(abs(an), 1e-3 * scale), (p, i, j, direction, fd, an)
+
+
+@pytest.mark.parametrize("stride", [1, 2, 3, 5, "sqrt", 12])
+def test_checkpointed_adjoint_matches_full_storage(stride):
+    """Checkpointing every s blocks recomputes the sweep in segments; the
+    gradients are those of the adjoint that keeps every block (uneven blocks,
+    non-orthogonal S, 12 blocks, 4 energies), to 1e-12 relative."""
+    from dpnegf_amd.negf.adjoint import transmission
+    from tests.test_gpu_rgf import _rand_se
+    dev = _dev()
+    sizes = [6, 9, 7, 8, 6, 10, 7, 9, 8, 6, 7, 9]
+    print(f"case checkpoint stride={stride} sizes={sizes}", flush=True)
+    hd, sd, hl, hu, sl, su = wl.random_btd(sizes, seed=3, overlap=True)
+    B = 4
+    sL, sR = _rand_se(B, sizes[0], 8).to(dev), _rand_se(B, sizes[-1], 9).to(dev)
+    E = _f64([-0.9, -0.2, 0.3, 1.1])
+    w = torch.tensor([0.5, -1.2, 0.8, 0.3], dtype=torch.float64, device=dev)
+    to = lambda xs: [x.to(dev) for x in xs]
+    grads, Ts = [], []
+    for ck in (0, stride):
+        params = [x.to(dev).requires_grad_(True) for x in hd + hu + hl]
+        K = len(sizes)
+        T = transmission(E, params[:K], params[K:2 * K - 1], params[2 * K - 1:], to(sd), to(su),
+                         to(sl), sL, sR, 1e-4, checkpoint=ck)
+        Ts.append(T.detach())
+        grads.append(torch.autograd.grad((T * w).sum(), params))
+    assert torch.equal(Ts[0], Ts[1])
+    scale = max(g.abs().max().item() for g in grads[0])
+    for a, b in zip(*grads):
+        assert (a - b).abs().max().item() < 1e-12 * scale
+
+
+def test_checkpoint_auto_keeps_every_block_when_it_fits():
+    from dpnegf_amd.negf.adjoint import checkpoint_stride
+    dev = _dev()
+    sizes = [8] * 9
+    assert checkpoint_stride(None, sizes, 4, dev) == 0
+    assert checkpoint_stride(None, sizes, 4, dev, mem_fraction=0.0) == 3
