@@ -199,8 +199,9 @@ __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
     }
 }
 
-template <int NMAX, int NT>
-__global__ __launch_bounds__(NT) void binv_kernel(cx<double>* A, int n, int64_t stride,
+// MIN_WAVES: minimum waves per SIMD the register allocation must allow.
+template <int NMAX, int NT, int MIN_WAVES = 1>
+__global__ __launch_bounds__(NT, MIN_WAVES) void binv_kernel(cx<double>* A, int n, int64_t stride,
                                                   int32_t* status) {
   constexpr int MAXR = gj_maxr(NMAX, NT);
   __shared__ GJScratch<double, NMAX> gjs;
@@ -575,6 +576,11 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
   if (n <= 16) hipLaunchKernelGGL((binv_kernel<16, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
   else if (n <= 32) hipLaunchKernelGGL((binv_kernel<32, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
   else if (n <= 64) hipLaunchKernelGGL((binv_kernel<64, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
+  // The sweep's n pivot steps (two workgroup barriers each) are latency-bound:
+  // up to 96, 512 threads with registers capped at 128 let two workgroups
+  // share a CU and fill each other's gaps.
+  else if (n <= 96)
+    hipLaunchKernelGGL((binv_kernel<96, 512, 4>), dim3(B), dim3(512), 0, st, A, n, s, status);
   else if (n <= GJ_NMAX_LARGE)
     hipLaunchKernelGGL((binv_kernel<GJ_NMAX_LARGE, 1024>), dim3(B), dim3(1024), 0, st, A, n, s, status);
   else if (n <= BLOCK_NMAX) {
