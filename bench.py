@@ -16,8 +16,14 @@ the same step (both leads, then the transmission sweep, here through the
 block-operation path for blocks above 64).  Its rate is per (E,k) point, so a
 few k-points give the same figure as 32; it is not the headline line.
 
+``--workload density`` measures the Ozaki density on the large-block device:
+8 orbitals/site, layer widths cycling 32/24/28/30 sites (blocks 256/192/224/240),
+24 layers, per k-point 160 poles plus the R term and a 512-node Gauss-Legendre
+window for a 0.1 V bias, every diagonal block.  Its rate is (E,k) points/s
+over those 673 points per k-point (``--kpoints``, default 1 here).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
-                       [--workload ribbon48|multiorbital] [--kpoints NK]
+                       [--workload ribbon48|multiorbital|density] [--kpoints NK]
 """
 import argparse
 import json
@@ -186,6 +192,58 @@ def bench_multiorbital(args):
     print(json.dumps(out))
 
 
+def bench_density(args):
+    """Ozaki density, blocks 256/192/224/240, 24 layers, args.kpoints k-points."""
+    from dpnegf_amd import workloads as wl
+    from dpnegf_amd.negf.density import Ozaki, ozaki_density
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    nk = args.kpoints
+    systems = [wl.multiorbital(widths=(32, 24, 28, 30), K=24, norb=8, kpoint=k / 16, seed=0).to(dev)
+               for k in range(nk)]
+    kBT, mu, mu_L, R, M, n_gauss = 0.025852, 0.0, 0.0, 1e6, 160, 512
+    vL, vR = 0.05, -0.05
+    oz = Ozaki(R, M, n_gauss)
+    fL = lambda x: 1 / (1 + torch.exp((x - (mu - vL)) / kBT))
+    fR = lambda x: 1 / (1 + torch.exp((x - (mu - vR)) / kBT))
+    out_dm = {}
+
+    def step():
+        tr = 0.0
+        for s in systems:
+            eq, neq = ozaki_density((s.hd, s.hu, s.hl, s.sd, s.su, s.sl), s.lead_L, s.lead_R, kBT,
+                                    mu_L, mu, R, oz.poles, oz.residues, 0.0, 1e-5, 1e-6, vL, vR,
+                                    n_gauss, fL, fR)
+            tr += float(sum(d.diagonal().sum() for d in eq)) + float(sum(d.diagonal().sum()
+                                                                          for d in neq))
+        out_dm["trace"] = tr
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    P = nk * (1 + len(oz.poles) + n_gauss)
+    sizes = systems[0].sizes
+    print(json.dumps({
+        "metric": METRIC, "value": round(P / dt, 3), "unit": "(E,k) points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * dt, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Ozaki density, multiorbital 8 orb/site, widths 32/24/28/30 (blocks "
+                               f"{'/'.join(str(n) for n in sizes[:4])}), 24 layers, seed 0, {nk} "
+                               f"k-point(s) x ({len(oz.poles)} poles + R term + {n_gauss} "
+                               f"Gauss-Legendre nodes, bias 0.1 V, kT 25.85 meV); step = Sigma_L + "
+                               f"Sigma_R + full-output RGF + pole / window sums, every diagonal "
+                               f"block", "points_per_step": P},
+        "density_trace": out_dm["trace"],
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,9 +253,10 @@ def main():
     ap.add_argument("--energies", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="ribbon48", choices=["ribbon48", "multiorbital"])
-    ap.add_argument("--kpoints", type=int, default=32,
-                    help="transverse k-points of the multiorbital workload")
+    ap.add_argument("--workload", default="ribbon48", choices=["ribbon48", "multiorbital", "density"])
+    ap.add_argument("--kpoints", type=int, default=None,
+                    help="transverse k-points of the multiorbital (default 32) or density (default 1) "
+                         "workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -208,7 +267,12 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "multiorbital":
+        args.kpoints = 32 if args.kpoints is None else args.kpoints
         bench_multiorbital(args)
+        return
+    if args.workload == "density":
+        args.kpoints = 1 if args.kpoints is None else args.kpoints
+        bench_density(args)
         return
 
     from dpnegf_amd.negf.recursive_green_cal import BTDSystems, rgf_transmission

@@ -139,8 +139,11 @@ def _run_full(z, arena, sizes, ortho, sigL, nL, sigR, nR, need_lc, s_in=None, s_
     out["greater"] = ((s_out, alloc(nsq), alloc(nsq), alloc(nx), alloc(nx))
                       if s_out is not None else None)
     status = torch.zeros(B, dtype=torch.int32, device=device)
-    nmax = max(sizes)
-    work = torch.empty(5 * B * nmax * nmax, dtype=C128, device=device)
+    # the library's own size: it includes the two-level inverse's workspace for blocks above 128
+    nwork = nat.load().dpnegf_rgf_full_workspace_elems(K, nat.host_i32(sizes), B)
+    if nwork < 0:
+        raise ValueError(nat.load().dpnegf_last_error().decode())
+    work = torch.empty(nwork, dtype=C128, device=device)
     args = [nat.stream_ptr(device), K, nat.host_i32(sizes), nat.ptr(arena), B, nat.ptr(z),
             nat.ptr(sigL), nL, nat.ptr(sigR), nR, int(ortho),
             nat.ptr(out["gl"]), nat.ptr(out["grd"]), nat.ptr(out["grl"]), nat.ptr(out["gru"]),

@@ -118,3 +118,29 @@ def test_checkpoint_stride_choices():
     assert checkpoint_stride(0, [8] * 10, 2, None) == 0
     with pytest.raises(ValueError):
         checkpoint_stride(-1, [8] * 10, 2, None)
+
+
+@pytest.mark.skipif(not os.path.exists(nat.lib_path()), reason="library not built")
+def test_full_output_workspace_covers_the_library_size(monkeypatch):
+    """The work buffer recursive_gf (and the density) hand to dpnegf_rgf_full
+    is at least what the library declares; above 128 orbitals that includes
+    the two-level inverse's 3 B n^2 elements (a 5 B n^2 buffer was written
+    past its end there).  The library call itself is intercepted."""
+    import torch
+    from dpnegf_amd.negf import recursive_green_cal as rgc
+    passed = []
+    seen = {}
+    monkeypatch.setattr(nat, "ptr", lambda t: passed.append(t) or ctypes.c_void_p(0))
+    monkeypatch.setattr(nat, "stream_ptr", lambda device=None: ctypes.c_void_p(0))
+    monkeypatch.setattr(nat, "call", lambda name, *args: seen.__setitem__(name, list(passed)))
+    lib = nat.load()
+    for sizes in ([256, 192, 240], [144, 168, 136], [8, 12], [130]):
+        B = 3
+        passed.clear()
+        z = torch.zeros(B, dtype=torch.complex128)
+        sig = torch.zeros(B, 4, 4, dtype=torch.complex128)
+        rgc._run_full(z, torch.zeros(1, dtype=torch.complex128), sizes, True, sig, 4, sig, 4, True)
+        work, status = seen["dpnegf_rgf_full"][-2:]
+        need = lib.dpnegf_rgf_full_workspace_elems(len(sizes), nat.host_i32(sizes), B)
+        assert status.numel() == B
+        assert work.numel() >= need, (sizes, work.numel(), need)

@@ -9,10 +9,7 @@ import torch
 
 from dpnegf_amd import _native as nat
 
-FAULT_SKIP = ("these cases ran in a GPU command whose machine ended with a GPU memory fault, "
-              "together with a matrix-core inverse kernel that has since been removed; the "
-              "faulting kernel was not identified, so they stay off the GPU")
-pytestmark = [pytest.mark.gpu, pytest.mark.skip(reason=FAULT_SKIP)]
+pytestmark = pytest.mark.gpu
 
 
 def _dev():

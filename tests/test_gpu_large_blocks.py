@@ -56,13 +56,7 @@ def test_blocked_transmission_uneven_large_blocks():
     assert np.abs(T - np.array(ref)).max() < 1e-10 * max(1.0, np.abs(ref).max()), (T, ref)
 
 
-FAULT_SKIP = ("a GPU memory fault surfaced in this case in the run where the autotuner's timing "
-              "trial first ran; its cause is not identified, so the case stays off the GPU")
-
-
-@pytest.mark.parametrize("sizes", [[256, 192, 240],
-                                   pytest.param([144, 168, 136],
-                                                marks=pytest.mark.skip(reason=FAULT_SKIP))])
+@pytest.mark.parametrize("sizes", [[256, 192, 240], [144, 168, 136]])
 def test_recursive_gf_blocks_up_to_256(sizes):
     """Blocks above 128 (two-level inverse): every output block of
     recursive_gf for uneven blocks with non-orthogonal S."""
@@ -104,7 +98,6 @@ def test_transmission_multiorbital_256_blocks():
     assert np.abs(T - np.array(ref)).max() < 1e-10 * max(1.0, np.abs(ref).max()), (T, ref)
 
 
-@pytest.mark.skip(reason=FAULT_SKIP)
 def test_autotuned_chunk_gives_identical_transmission():
     """The block-operation sweep computes every point independently, so the
     chunk the autotuner picks changes nothing in T: bit-identical to fixed
