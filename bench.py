@@ -219,12 +219,15 @@ def bench_density(args):
                                                                           for d in neq))
         out_dm["trace"] = tr
 
-    for _ in range(args.warmup):
+    for i in range(args.warmup):
         step()
+        torch.cuda.synchronize()
+        print(f"density warm-up step {i + 1} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
         step()
+        print(f"density step {i + 1} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     P = nk * (1 + len(oz.poles) + n_gauss)
