@@ -292,7 +292,8 @@ def bench_adjoint(args):
         dt = (time.perf_counter() - t0) / args.steps
         lines[str(mode)] = {"ms_per_step": round(1e3 * dt, 3), "points_per_s": round(nE / dt, 3),
                             "peak_hbm_GB": round(torch.cuda.max_memory_allocated() / 1e9, 3),
-                            "J": float(J), "grad_norm": float(sum(x.abs().square().sum() for x in g))}
+                            "J": float(J.detach()),
+                            "grad_norm": float(sum(x.abs().square().sum() for x in g))}
         print(f"adjoint mode {mode} done", file=sys.stderr, flush=True)
     full = lines["0"]
     print(json.dumps({
