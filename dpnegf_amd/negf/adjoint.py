@@ -21,7 +21,7 @@ import math
 import torch
 
 from .. import _native as nat
-from ._btd import C128, block_sizes, is_orthogonal, pack_arena, split_blocks
+from ._btd import C128, split_blocks
 from .recursive_green_cal import BTDSystems, rgf_transmission
 
 

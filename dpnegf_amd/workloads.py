@@ -17,7 +17,7 @@ the left and ``hLL = hDL = H01`` on the right.
 All energies are in eV; S = I and S_offdiag = 0 unless stated.
 """
 from dataclasses import dataclass, field
-from typing import List, Optional, Tuple
+from typing import List, Tuple
 
 import numpy as np
 import torch

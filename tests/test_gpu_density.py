@@ -3,8 +3,6 @@ oracle: Ozaki poles, oracle Lopez-Sancho self-energies and oracle RGF blocks,
 summed with the formula of ``density.py:164-195`` (pinned in
 test_oracle_ozaki_density.py).  Bar: 1e-10 relative, FP64.
 """
-import math
-
 import numpy as np
 import pytest
 import torch

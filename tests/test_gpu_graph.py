@@ -2,7 +2,6 @@
 several replays with eager work and synchronisation between them, then an
 SCF-like update of the Hamiltonian (a potential on the device layers) and a
 replay against the eager result for the new Hamiltonian.  Bar 1e-12 of max T."""
-import numpy as np
 import pytest
 import torch
 

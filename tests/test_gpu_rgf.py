@@ -3,8 +3,6 @@
 Bar: relative 1e-10 on every output block / T(E) (FP64 path).  The oracle is
 the reference restatement in ``oracle/`` (pinned by tests/test_oracle_*.py).
 """
-import math
-
 import numpy as np
 import pytest
 import torch
