@@ -271,13 +271,30 @@ __global__ void badd_identity_kernel(cx<double>* D, int n, int B) {
 }
 
 // out[i][k] = sum_b conj(in[b][k][i]);  in: [B, m, n], out: [n, m]
-__global__ void breduce_conjt_kernel(cx<double>* out, const cx<double>* in, int m, int n, int B) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m * n) return;
-  const int k = e / n, i = e % n;  // in element (k, i)
+// out[i][k] = conj(sum_b in[b][k][i]) for an m x n block: a gradient block
+// summed over the batch, transposed and conjugated.  A 256-thread workgroup
+// takes RED_ELEMS consecutive elements; thread t sums element t % RED_ELEMS
+// over the batch entries b = t / RED_ELEMS + RED_PHASES j, so one wave reads
+// four 256-byte runs per step and a 64 x 64 block spreads over 256
+// workgroups.  The partial sums of an element are added in a fixed order.
+constexpr int RED_ELEMS = 16, RED_PHASES = 16;
+__global__ __launch_bounds__(256) void breduce_conjt_kernel(cx<double>* out, const cx<double>* in,
+                                                            int m, int n, int B) {
+  __shared__ cx<double> part[RED_PHASES][RED_ELEMS];
+  const int64_t mn = (int64_t)m * n;
+  const int el = threadIdx.x % RED_ELEMS, ph = threadIdx.x / RED_ELEMS;
+  const int64_t e = (int64_t)blockIdx.x * RED_ELEMS + el;  // in element (k, i) = (e / n, e % n)
   cx<double> acc{0, 0};
-  for (int b = 0; b < B; ++b) acc = acc + in[(int64_t)b * m * n + e];
-  out[(int64_t)i * m + k] = conj(acc);
+  if (e < mn)
+    for (int b = ph; b < B; b += RED_PHASES) acc = acc + in[(int64_t)b * mn + e];
+  part[ph][el] = acc;
+  __syncthreads();
+  if (threadIdx.x < RED_ELEMS && e < mn) {
+    cx<double> sum = part[0][el];
+    for (int p = 1; p < RED_PHASES; ++p) sum = sum + part[p][el];
+    const int k = (int)(e / n), i = (int)(e % n);
+    out[(int64_t)i * m + k] = conj(sum);
+  }
 }
 
 // Gamma = i (Sigma - Sigma^H) padded into an n x n block (top-left or
@@ -575,7 +592,8 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
   const int64_t s = (int64_t)n * n;
   if (n <= 16) hipLaunchKernelGGL((binv_kernel<16, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
   else if (n <= 32) hipLaunchKernelGGL((binv_kernel<32, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
-  else if (n <= 64) hipLaunchKernelGGL((binv_kernel<64, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
+  else if (n <= 64)  // 8 rows per thread and 71 registers: three workgroups per CU
+    hipLaunchKernelGGL((binv_kernel<64, 512, 4>), dim3(B), dim3(512), 0, st, A, n, s, status);
   // The sweep's n pivot steps (two workgroup barriers each) are latency-bound:
   // up to 96, 512 threads with registers capped at 128 let two workgroups
   // share a CU and fill each other's gaps.
@@ -1106,8 +1124,8 @@ hipError_t rgf_grad(const RgfGradArgs& g, hipStream_t st) {
            zero, C, (int64_t)nK * n0, n0, g.z));
   // first block row: G_{0,0} = grd_0; G_{0,b} = R_b G_bb, R_1 = g_0 U_0, R_{b+1} = R_b g_b U_b
   auto reduce = [&](cx<double>* out, const cx<double>* in, int m_, int n_) {
-    hipLaunchKernelGGL(breduce_conjt_kernel, dim3((m_ * n_ + 255) / 256), dim3(256), 0, st, out,
-                       in, m_, n_, B);
+    hipLaunchKernelGGL(breduce_conjt_kernel, dim3((m_ * n_ + RED_ELEMS - 1) / RED_ELEMS), dim3(256),
+                       0, st, out, in, m_, n_, B);
     return hipGetLastError();
   };
   const cx<double>* g0b = grd(0);  // G_{0,0}
@@ -1274,8 +1292,8 @@ hipError_t rgf_grad_ckpt(const RgfGradArgs& g, int stride, hipStream_t st) {
                 (int64_t)n0 * n1, n1, g.z);
   };
   auto reduce = [&](cx<double>* out, const cx<double>* in, int m_, int n_) {
-    hipLaunchKernelGGL(breduce_conjt_kernel, dim3((m_ * n_ + 255) / 256), dim3(256), 0, st, out, in,
-                       m_, n_, B);
+    hipLaunchKernelGGL(breduce_conjt_kernel, dim3((m_ * n_ + RED_ELEMS - 1) / RED_ELEMS), dim3(256),
+                       0, st, out, in, m_, n_, B);
     return hipGetLastError();
   };
 
