@@ -144,3 +144,14 @@ def test_full_output_workspace_covers_the_library_size(monkeypatch):
         need = lib.dpnegf_rgf_full_workspace_elems(len(sizes), nat.host_i32(sizes), B)
         assert status.numel() == B
         assert work.numel() >= need, (sizes, work.numel(), need)
+
+
+def test_entry_module_helpers_run_without_a_gpu():
+    """smoke()'s host-side helpers resolve their imports (the GPU part runs at
+    round end on the device)."""
+    import importlib
+    import torch
+    entry = importlib.import_module("__graft_entry__")
+    t = entry._f64([0.5, -1.0])
+    assert t.dtype == torch.float64 and t.tolist() == [0.5, -1.0]
+    assert callable(entry.build) and callable(entry.smoke)
