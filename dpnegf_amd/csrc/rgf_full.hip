@@ -473,6 +473,17 @@ __global__ void ls_mark_kernel(const int32_t* done, int32_t* status, int32_t* it
 }
 
 // out = a - b elementwise
+// a += t and b += t: the decimation's beta X_a, formed once, enters eps and
+// eps_s (surface_green.py:196-200).
+__global__ void badd2_kernel(cx<double>* a, cx<double>* b, const cx<double>* t, int64_t count) {
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < count;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const cx<double> v = t[x];
+    a[x] = a[x] + v;
+    b[x] = b[x] + v;
+  }
+}
+
 __global__ void bsub_kernel(cx<double>* out, const cx<double>* a, const cx<double>* b, int64_t count) {
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < count;
        x += (int64_t)gridDim.x * blockDim.x)
@@ -971,10 +982,12 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
       CHK(gemm(st, B, n, n, n, M(X1), M(sol[q]), mone, one, X2, nn, n, nullptr));
       CHK(gemm(st, B, n, n, n, M(W), M(X2), one, one, sol[q], nn, n, nullptr));
     }
-    // eps += alpha X_b + beta X_a ; eps_s += beta X_a ; alpha <- alpha X_a ; beta <- beta X_b
+    // eps += alpha X_b + beta X_a ; eps_s += beta X_a (beta X_a formed once, as the
+    // reference does) ; alpha <- alpha X_a ; beta <- beta X_b
     CHK(gemm(st, B, n, n, n, M(al), M(Bi), one, one, eps, nn, n, nullptr));
-    CHK(gemm(st, B, n, n, n, M(be), M(Ai), one, one, eps, nn, n, nullptr));
-    CHK(gemm(st, B, n, n, n, M(be), M(Ai), one, one, epss, nn, n, nullptr));
+    CHK(gemm(st, B, n, n, n, M(be), M(Ai), one, zero, Tm, nn, n, nullptr));
+    hipLaunchKernelGGL(badd2_kernel, dim3(gridn), dim3(256), 0, st, eps, epss, Tm, (int64_t)B * nn);
+    CHK(hipGetLastError());
     CHK(gemm(st, B, n, n, n, M(al), M(Ai), one, zero, Tm, nn, n, nullptr));
     std::swap(al, Tm);
     CHK(gemm(st, B, n, n, n, M(be), M(Bi), one, zero, Tm, nn, n, nullptr));
