@@ -178,6 +178,7 @@ __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
   else if (RX == 1 && CY == 1) run(I1(), I1());
   else run(I0(), I0());  // loads and barriers only
   cx<double>* C = a.C + (int64_t)b * a.cstride;
+  const cx<double>* Cin = a.Cin + (int64_t)b * a.cstride;
   const cx<double> alpha = a.alpha, beta = a.beta;
   const bool use_beta = (beta.re != 0.0 || beta.im != 0.0);
 #pragma unroll
@@ -190,10 +191,10 @@ __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
       for (int r = 0; r < 4; ++r) {
         const int i = ib + mfma<double>::row(lk, r);
         if (i < a.m && j < a.n) {
-          cx<double>* c = C + (int64_t)i * a.ldc + j;
+          const int64_t o = (int64_t)i * a.ldc + j;
           cx<double> out = alpha * cx<double>{cr[x][y][r], ci[x][y][r]};
-          if (use_beta) out = out + beta * (*c);
-          *c = out;
+          if (use_beta) out = out + beta * Cin[o];
+          C[o] = out;
         }
       }
     }
@@ -333,9 +334,10 @@ __global__ void bweighted_re_sum_kernel(double* out, const cx<double>* in, const
   }
 }
 
+// C = alpha op(A) op(B) + beta Cin over a batch; Cin (laid out as C) defaults to C.
 static hipError_t gemm(hipStream_t st, int B, int m, int n, int k, Opnd A, Opnd Bo,
                        cx<double> alpha, cx<double> beta, cx<double>* C, int64_t cstride,
-                       int ldc, const cx<double>* z);
+                       int ldc, const cx<double>* z, const cx<double>* Cin = nullptr);
 static Opnd mat(const cx<double>* h, int ld, int64_t bstride, int op, const cx<double>* s,
                 int64_t sstride);
 
@@ -553,13 +555,13 @@ __global__ void ls_coupling_kernel(cx<double>* out, const cx<double>* lead, int6
 
 static hipError_t gemm(hipStream_t st, int B, int m, int n, int k, Opnd A, Opnd Bo,
                        cx<double> alpha, cx<double> beta, cx<double>* C, int64_t cstride,
-                       int ldc, const cx<double>* z) {
+                       int ldc, const cx<double>* z, const cx<double>* Cin) {
   if (m <= 0 || n <= 0) return hipSuccess;
   if (k <= 0) return hipErrorInvalidValue;  // the kernel clamps indices into [0, k)
   BGemmArgs a;
   a.m = m; a.n = n; a.k = k; a.B = B;
   a.A = A; a.B_ = Bo; a.alpha = alpha; a.beta = beta;
-  a.C = C; a.cstride = cstride; a.ldc = ldc; a.z = z;
+  a.C = C; a.Cin = Cin ? Cin : C; a.cstride = cstride; a.ldc = ldc; a.z = z;
   dim3 grid(((m + 63) / 64) * ((n + 63) / 64), B);
   const bool sa = A.s != nullptr, sb = Bo.s != nullptr;
   if (sa && sb) hipLaunchKernelGGL((bgemm_lds_kernel<true, true>), grid, dim3(256), 0, st, a);
@@ -597,9 +599,10 @@ static hipError_t block_copy(hipStream_t st, cx<double>* D, int ldd, int64_t sd,
 // anti-Hermitian part is positive definite, so A11 and S are nonsingular --
 // but they can be far worse conditioned than A, so the assembled inverse takes
 // one Newton step in FP64, X <- X + X (I - A X), which squares its relative
-// error.  ws: inverse_workspace(n, B) elements.
+// error.  A_copy: an existing copy of A (same layout) for that step's residual,
+// which saves copying A into the workspace.  ws: inverse_workspace(n, B) elements.
 static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* status,
-                          cx<double>* ws) {
+                          cx<double>* ws, const cx<double>* A_copy = nullptr) {
   const int64_t s = (int64_t)n * n;
   if (n <= 16) hipLaunchKernelGGL((binv_kernel<16, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
   else if (n <= 32) hipLaunchKernelGGL((binv_kernel<32, 256>), dim3(B), dim3(256), 0, st, A, n, s, status);
@@ -616,8 +619,9 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
     if (!ws) return hipErrorInvalidValue;
     const int h1 = n / 2, h2 = n - h1;
     const int64_t s1 = (int64_t)h1 * h1, s2 = (int64_t)h2 * h2, s12 = (int64_t)h1 * h2;
-    cx<double>* A0 = ws;             // A as given             [n x n]
-    cx<double>* Xc = A0 + B * s;     // copy of the inverse    [n x n]
+    cx<double>* A0w = ws;            // A as given             [n x n]
+    const cx<double>* A0 = A0w;
+    cx<double>* Xc = A0w + B * s;    // copy of the inverse    [n x n]
     cx<double>* X = Xc + B * s;      // A11^-1   [h1 x h1]  (then I - A X, [n x n])
     cx<double>* Y = X + B * s1;      // S^-1     [h2 x h2]
     cx<double>* T1 = Y + B * s2;     // X A12    [h1 x h2]
@@ -629,7 +633,8 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
     cx<double>* A12 = A + h1;
     cx<double>* A21 = A + (int64_t)h1 * n;
     cx<double>* A22 = A21 + h1;
-    CHK(block_copy(st, A0, n, s, A, n, s, n, n, B));
+    if (A_copy) A0 = A_copy;
+    else CHK(block_copy(st, A0w, n, s, A, n, s, n, n, B));
     CHK(block_copy(st, X, h1, s1, A, n, s, h1, h1, B));
     CHK(inverse(st, X, h1, B, status, nullptr));
     CHK(gemm(st, B, h1, h2, h1, M(X, h1, s1), M(A12, n, s), one, zero, T1, s12, h2, nullptr));
@@ -973,13 +978,13 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
     // through the inverse, refined once in FP64, X += W^-1 (rhs - W X).  Without the
     // refinement, W^-1 times rhs loses the Dyson residual near band edges at small eta.
     CHK(hipMemcpyAsync(X1, W, sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
-    CHK(inverse(st, W, n, B, status, iws));
+    CHK(inverse(st, W, n, B, status, iws, X1));
     cx<double>* rhs[2] = {al, be};
     cx<double>* sol[2] = {Ai, Bi};
     for (int q = 0; q < 2; ++q) {
       CHK(gemm(st, B, n, n, n, M(W), M(rhs[q]), one, zero, sol[q], nn, n, nullptr));
-      CHK(hipMemcpyAsync(X2, rhs[q], sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
-      CHK(gemm(st, B, n, n, n, M(X1), M(sol[q]), mone, one, X2, nn, n, nullptr));
+      // X2 = rhs - W X
+      CHK(gemm(st, B, n, n, n, M(X1), M(sol[q]), mone, one, X2, nn, n, nullptr, rhs[q]));
       CHK(gemm(st, B, n, n, n, M(W), M(X2), one, one, sol[q], nn, n, nullptr));
     }
     // eps += alpha X_b + beta X_a ; eps_s += beta X_a (beta X_a formed once, as the

@@ -30,6 +30,7 @@ struct BGemmArgs {
   Opnd A, B_;
   cx<double> alpha, beta;
   cx<double>* C;
+  const cx<double>* Cin;  // source of the beta term, laid out as C (often C itself)
   int64_t cstride;
   int ldc;
   const cx<double>* z;
