@@ -36,31 +36,35 @@ __device__ __forceinline__ OpView op_view(const Opnd& o) {
           (o.op == OP_H || o.op == OP_C) ? -1.0 : 1.0};
 }
 
-// Batched C = alpha op(A) op(B) + beta C: a 64 x 64 output tile per
-// workgroup, wave w owning the 2 x 2 block of 16 x 16 tiles at rows 32 (w / 2),
-// columns 32 (w % 2).  K is staged through LDS in chunks of KC = 16: the 256
-// threads load the A chunk (64 x 16, thread t at column k = t % 16 of rows
-// t / 16 + 16 u) and the B chunk (16 x 64, row t / 16, columns t % 16 + 16 u),
-// so sixteen consecutive lanes read sixteen consecutive elements.  Double
-// buffered: the next chunk's raw h (and s) values are loaded into registers
-// before this chunk's matrix-core steps and turned into operand values (h - z s,
-// the conjugation, zeros outside the matrix) only when they are stored to LDS
-// after those steps, so no arithmetic waits on the loads in between.  Indices
-// are clamped, so every load is in bounds; a chunk past K contributes zeros.
-// A's LDS rows are padded to 17 elements so that the fragment reads (16 rows at
-// one k) spread over the banks.  Matrix-core work is skipped, with wave-uniform
-// control flow, for 16 x 16 fragment tiles outside the output and for k-steps
-// past K, so edge tiles cost what they cover at 16-element granularity.
-template <bool SA, bool SB>
+// Batched C = alpha op(A) op(B) + beta C: a TM x TN = 32 FR x 32 FC output tile
+// per workgroup (FR, FC in {1, 2}), wave w owning the FR x FC block of 16 x 16
+// tiles at rows 16 FR (w / 2), columns 16 FC (w % 2).  The host picks FR = 1
+// when the last 64-row tile would be at most half full (m mod 64 in (0, 32]),
+// and FC likewise, so the padding is at most 31 rows or columns.  K is staged
+// through LDS in chunks of KC = 16: the 256 threads load the A chunk (TM x 16,
+// thread t at column k = t % 16 of rows t / 16 + 16 u) and the B chunk
+// (16 x TN, row t / 16, columns t % 16 + 16 u), so sixteen consecutive lanes
+// read sixteen consecutive elements.  Double buffered: the next chunk's raw h
+// (and s) values are loaded into registers before this chunk's matrix-core
+// steps and turned into operand values (h - z s, the conjugation, zeros outside
+// the matrix) only when they are stored to LDS after those steps, so no
+// arithmetic waits on the loads in between.  Indices are clamped, so every load
+// is in bounds; a chunk past K contributes zeros.  A's LDS rows are padded to
+// 17 elements so that the fragment reads (16 rows at one k) spread over the
+// banks.  Matrix-core work is skipped, with wave-uniform control flow, for
+// 16 x 16 fragment tiles outside the output and for k-steps past K.
+template <bool SA, bool SB, int FR, int FC>
 __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
+  static_assert((FR == 1 || FR == 2) && (FC == 1 || FC == 2), "fragments per wave");
   typedef mfma<double>::v4 v4;
-  constexpr int KC = 16, LDAS = KC + 1, LDBS = 64;
-  __shared__ cx<double> As[2][64 * LDAS];
+  constexpr int TM = 32 * FR, TN = 32 * FC, UA = 2 * FR, UB = 2 * FC;
+  constexpr int KC = 16, LDAS = KC + 1, LDBS = TN;
+  __shared__ cx<double> As[2][TM * LDAS];
   __shared__ cx<double> Bs[2][KC * LDBS];
-  const int tn = (a.n + 63) / 64;
+  const int tn = (a.n + TN - 1) / TN;
   const int ti = blockIdx.x / tn, tj = blockIdx.x % tn;
   const int b = blockIdx.y;
-  const int i0 = ti * 64, j0 = tj * 64;
+  const int i0 = ti * TM, j0 = tj * TN;
   const cx<double> zb = a.z ? a.z[b] : cx<double>{0, 0};
   const OpView A = op_view(a.A), Bv = op_view(a.B_);
   const cx<double>* const Ah = A.h + (int64_t)b * A.bstride;
@@ -72,26 +76,33 @@ __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
   const int li = lane & 15, lk = lane >> 4;
   const int K = a.k;
   // load assignment
-  const int ak = t & 15, ar = t >> 4;  // A: column ak of rows ar + 16 u
-  const int bk = t >> 4, bc = t & 15;  // B: row bk, columns bc + 16 u
-  int64_t aoff[4], boff[4];
-  bool arok[4], bcok[4];
+  const int ak = t & 15, ar = t >> 4;  // A: column ak of rows ar + 16 u, u < UA
+  const int bk = t >> 4, bc = t & 15;  // B: row bk, columns bc + 16 u, u < UB
+  int64_t aoff[UA], boff[UB];
+  bool arok[UA], bcok[UB];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int r = i0 + ar + 16 * u, c = j0 + bc + 16 * u;
+  for (int u = 0; u < UA; ++u) {
+    const int r = i0 + ar + 16 * u;
     arok[u] = r < a.m;
-    bcok[u] = c < a.n;
     aoff[u] = (int64_t)(arok[u] ? r : a.m - 1) * A.rs;
+  }
+#pragma unroll
+  for (int u = 0; u < UB; ++u) {
+    const int c = j0 + bc + 16 * u;
+    bcok[u] = c < a.n;
     boff[u] = (int64_t)(bcok[u] ? c : a.n - 1) * Bv.cs;
   }
-  cx<double> ra[4], rb[4], rsa[4], rsb[4];
+  cx<double> ra[UA], rb[UB], rsa[UA], rsb[UB];
   auto load_chunk = [&](int k0) __attribute__((always_inline)) {
     const int64_t ka = (int64_t)min(k0 + ak, K - 1) * A.cs;
     const int64_t kb = (int64_t)min(k0 + bk, K - 1) * Bv.rs;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < UA; ++u) {
       ra[u] = Ah[aoff[u] + ka];
       if (SA) rsa[u] = Asp[aoff[u] + ka];
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
       rb[u] = Bh[boff[u] + kb];
       if (SB) rsb[u] = Bsp[boff[u] + kb];
     }
@@ -99,31 +110,37 @@ __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
   auto store_chunk = [&](int k0, int buf) __attribute__((always_inline)) {
     const bool kaok = k0 + ak < K, kbok = k0 + bk < K;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      cx<double> va = ra[u], vb = rb[u];
+    for (int u = 0; u < UA; ++u) {
+      cx<double> va = ra[u];
       if (SA) va = va - zb * rsa[u];
-      if (SB) vb = vb - zb * rsb[u];
-      const bool oa = arok[u] && kaok, ob = bcok[u] && kbok;
+      const bool oa = arok[u] && kaok;
       As[buf][(ar + 16 * u) * LDAS + ak] = {oa ? va.re : 0.0, oa ? A.csign * va.im : 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      cx<double> vb = rb[u];
+      if (SB) vb = vb - zb * rsb[u];
+      const bool ob = bcok[u] && kbok;
       Bs[buf][bk * LDBS + bc + 16 * u] = {ob ? vb.re : 0.0, ob ? Bv.csign * vb.im : 0.0};
     }
   };
-  v4 cr[2][2], ci[2][2];
+  v4 cr[FR][FC], ci[FR][FC];
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int x = 0; x < FR; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) cr[x][y] = ci[x][y] = v4{0, 0, 0, 0};
-  const int rbase = 32 * (w / 2), cbase = 32 * (w % 2);
+    for (int y = 0; y < FC; ++y) cr[x][y] = ci[x][y] = v4{0, 0, 0, 0};
+  const int rbase = 16 * FR * (w / 2), cbase = 16 * FC * (w % 2);
   // NK matrix-core k-steps of the chunk in LDS buffer buf, for the RX x CY
   // fragment tiles of this wave that overlap the output (the others would only
   // multiply the zero padding).
   auto ksteps = [&](auto RXc, auto CYc, auto NKc, int buf) __attribute__((always_inline)) {
     constexpr int RX = decltype(RXc)::value, CY = decltype(CYc)::value;
     constexpr int NK = decltype(NKc)::value;
+    static_assert(RX <= FR && CY <= FC, "fragments beyond the wave's block");
 #pragma unroll
     for (int kk = 0; kk < NK; ++kk) {
       const int kl = 4 * kk + lk;
-      cx<double> pa[2], pb[2];
+      cx<double> pa[FR], pb[FC];
 #pragma unroll
       for (int x = 0; x < RX; ++x) pa[x] = As[buf][(rbase + 16 * x + li) * LDAS + kl];
 #pragma unroll
@@ -164,27 +181,33 @@ __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
   load_chunk(0);
   store_chunk(0, 0);
   __syncthreads();
-  // fragment tiles of this wave inside the output, wave-uniform
+  // fragment tiles of this wave inside the output (at most FR x FC), wave-uniform
   const int wr = __builtin_amdgcn_readfirstlane(i0 + rbase);
   const int wc = __builtin_amdgcn_readfirstlane(j0 + cbase);
-  const int RX = wr + 16 < a.m ? 2 : (wr < a.m ? 1 : 0);
-  const int CY = wc + 16 < a.n ? 2 : (wc < a.n ? 1 : 0);
+  const int RX = wr >= a.m ? 0 : (FR == 2 && wr + 16 < a.m ? 2 : 1);
+  const int CY = wc >= a.n ? 0 : (FC == 2 && wc + 16 < a.n ? 2 : 1);
   typedef std::integral_constant<int, 0> I0;
   typedef std::integral_constant<int, 1> I1;
   typedef std::integral_constant<int, 2> I2;
-  if (RX == 2 && CY == 2) run(I2(), I2());
-  else if (RX == 2 && CY == 1) run(I2(), I1());
-  else if (RX == 1 && CY == 2) run(I1(), I2());
-  else if (RX == 1 && CY == 1) run(I1(), I1());
-  else run(I0(), I0());  // loads and barriers only
+  if (RX == 0 || CY == 0) {
+    run(I0(), I0());  // loads and barriers only
+  } else if (RX == 1 && CY == 1) {
+    run(I1(), I1());
+  } else if (RX == 1) {  // CY == 2
+    if constexpr (FC == 2) run(I1(), I2());
+  } else if (CY == 1) {  // RX == 2
+    if constexpr (FR == 2) run(I2(), I1());
+  } else {
+    if constexpr (FR == 2 && FC == 2) run(I2(), I2());
+  }
   cx<double>* C = a.C + (int64_t)b * a.cstride;
   const cx<double>* Cin = a.Cin + (int64_t)b * a.cstride;
   const cx<double> alpha = a.alpha, beta = a.beta;
   const bool use_beta = (beta.re != 0.0 || beta.im != 0.0);
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int x = 0; x < FR; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) {
+    for (int y = 0; y < FC; ++y) {
       const int ib = i0 + rbase + 16 * x;
       const int j = j0 + cbase + 16 * y + li;
 #pragma unroll
@@ -562,12 +585,24 @@ static hipError_t gemm(hipStream_t st, int B, int m, int n, int k, Opnd A, Opnd 
   a.m = m; a.n = n; a.k = k; a.B = B;
   a.A = A; a.B_ = Bo; a.alpha = alpha; a.beta = beta;
   a.C = C; a.Cin = Cin ? Cin : C; a.cstride = cstride; a.ldc = ldc; a.z = z;
-  dim3 grid(((m + 63) / 64) * ((n + 63) / 64), B);
+  // 32-wide tiles in a dimension whose last 64-wide tile would be at most half full
+  const bool fr1 = m % 64 != 0 && m % 64 <= 32, fc1 = n % 64 != 0 && n % 64 <= 32;
+  const int tm = fr1 ? 32 : 64, tn = fc1 ? 32 : 64;
+  const dim3 grid(((m + tm - 1) / tm) * ((n + tn - 1) / tn), B);
   const bool sa = A.s != nullptr, sb = Bo.s != nullptr;
-  if (sa && sb) hipLaunchKernelGGL((bgemm_lds_kernel<true, true>), grid, dim3(256), 0, st, a);
-  else if (sa) hipLaunchKernelGGL((bgemm_lds_kernel<true, false>), grid, dim3(256), 0, st, a);
-  else if (sb) hipLaunchKernelGGL((bgemm_lds_kernel<false, true>), grid, dim3(256), 0, st, a);
-  else hipLaunchKernelGGL((bgemm_lds_kernel<false, false>), grid, dim3(256), 0, st, a);
+  auto launch = [&](auto FRc, auto FCc) {
+    constexpr int FR = decltype(FRc)::value, FC = decltype(FCc)::value;
+    if (sa && sb) hipLaunchKernelGGL((bgemm_lds_kernel<true, true, FR, FC>), grid, dim3(256), 0, st, a);
+    else if (sa) hipLaunchKernelGGL((bgemm_lds_kernel<true, false, FR, FC>), grid, dim3(256), 0, st, a);
+    else if (sb) hipLaunchKernelGGL((bgemm_lds_kernel<false, true, FR, FC>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((bgemm_lds_kernel<false, false, FR, FC>), grid, dim3(256), 0, st, a);
+  };
+  typedef std::integral_constant<int, 1> I1;
+  typedef std::integral_constant<int, 2> I2;
+  if (fr1 && fc1) launch(I1(), I1());
+  else if (fr1) launch(I1(), I2());
+  else if (fc1) launch(I2(), I1());
+  else launch(I2(), I2());
   return hipGetLastError();
 }
 
