@@ -21,7 +21,7 @@ STATUS_REFINE_FALLBACK = 16
 
 TRANSMISSION_NMAX = 64
 LEAD_NMAX = 48
-FULL_NMAX = 256
+FULL_NMAX = 1024
 
 _lib = None
 

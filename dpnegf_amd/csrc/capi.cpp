@@ -15,6 +15,7 @@
 #include "rgf_kernels.hpp"
 
 using negf::cx;
+static_assert(DPNEGF_FULL_NMAX == negf::BLOCK_NMAX, "block limit of the batched path");
 
 namespace {
 thread_local std::string g_err;

@@ -626,8 +626,9 @@ static hipError_t block_copy(hipStream_t st, cx<double>* D, int ldd, int64_t sd,
 
 // In-place inverse of B contiguous n x n matrices.  n <= GJ_NMAX_LARGE: one
 // workgroup per matrix, Gauss-Jordan with partial pivoting.  Larger n (up to
-// BLOCK_NMAX): two diagonal halves h1 = n/2, h2 = n - h1, each inverted with
-// partial pivoting, joined through the Schur complement S = A22 - A21 A11^-1 A12:
+// BLOCK_NMAX): two diagonal halves h1 = n/2, h2 = n - h1, each inverted by this
+// same function (recursively, until the halves are at most GJ_NMAX_LARGE),
+// joined through the Schur complement S = A22 - A21 A11^-1 A12:
 //   A^-1 = [[X + X A12 Y A21 X, -X A12 Y], [-Y A21 X, Y]],  X = A11^-1, Y = S^-1.
 // No pivoting across the halves: every matrix inverted on this path is
 // z S - H - Sigma (or a decimation W = z S - eps) with eta > 0, whose
@@ -636,6 +637,9 @@ static hipError_t block_copy(hipStream_t st, cx<double>* D, int ldd, int64_t sd,
 // one Newton step in FP64, X <- X + X (I - A X), which squares its relative
 // error.  A_copy: an existing copy of A (same layout) for that step's residual,
 // which saves copying A into the workspace.  ws: inverse_workspace(n, B) elements.
+// A half above GJ_NMAX_LARGE takes the region Xc below as its own workspace:
+// Xc is not used until the Newton step, and it holds B n^2 elements, at least
+// the 3 B ceil(n/2)^2 a half needs for n >= 7.
 static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* status,
                           cx<double>* ws, const cx<double>* A_copy = nullptr) {
   const int64_t s = (int64_t)n * n;
@@ -671,11 +675,11 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
     if (A_copy) A0 = A_copy;
     else CHK(block_copy(st, A0w, n, s, A, n, s, n, n, B));
     CHK(block_copy(st, X, h1, s1, A, n, s, h1, h1, B));
-    CHK(inverse(st, X, h1, B, status, nullptr));
+    CHK(inverse(st, X, h1, B, status, Xc));
     CHK(gemm(st, B, h1, h2, h1, M(X, h1, s1), M(A12, n, s), one, zero, T1, s12, h2, nullptr));
     CHK(block_copy(st, Y, h2, s2, A22, n, s, h2, h2, B));
     CHK(gemm(st, B, h2, h2, h1, M(A21, n, s), M(T1, h2, s12), mone, one, Y, s2, h2, nullptr));
-    CHK(inverse(st, Y, h2, B, status, nullptr));
+    CHK(inverse(st, Y, h2, B, status, Xc));
     CHK(gemm(st, B, h2, h1, h1, M(A21, n, s), M(X, h1, s1), one, zero, T2, s12, h1, nullptr));
     CHK(gemm(st, B, h1, h2, h2, M(T1, h2, s12), M(Y, h2, s2), mone, zero, A12, s, n, nullptr));
     CHK(gemm(st, B, h2, h1, h2, M(Y, h2, s2), M(T2, h1, s12), mone, zero, A21, s, n, nullptr));

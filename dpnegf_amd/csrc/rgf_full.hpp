@@ -6,10 +6,10 @@ namespace negf {
 
 enum { OP_N = 0, OP_H = 1, OP_T = 2, OP_C = 3 };
 // One-workgroup batched Gauss-Jordan inverse up to GJ_NMAX_LARGE (a 1024-thread
-// register-resident sweep; beyond 128 its column stripes spill), two-level
-// Schur-complement inverse with halves <= 128 above that, up to BLOCK_NMAX.
+// register-resident sweep; beyond 128 its column stripes spill), recursive
+// Schur-complement inverse over diagonal halves above that, up to BLOCK_NMAX.
 constexpr int GJ_NMAX_LARGE = 128;
-constexpr int BLOCK_NMAX = 256;
+constexpr int BLOCK_NMAX = 1024;
 // Workspace of the batched inverse for n x n blocks (0 up to GJ_NMAX_LARGE).
 int64_t inverse_workspace(int n, int B);
 // In-place inverse of B contiguous n x n blocks, 1 <= n <= GJ_NMAX_LARGE.

@@ -12,7 +12,8 @@ prefix of the caller's own system (the per-block cost is what the chunk size
 changes), keeps the fastest per (device, block sizes, overlap pattern) for the
 life of the process, and returns it.  Timing runs on the caller's current
 stream.  The sweep uses it only when ``DPNEGF_AUTOTUNE=1`` (``enabled()``);
-otherwise the chunk is DEFAULT_CHUNK.
+otherwise the chunk is ``default_chunk``: DEFAULT_CHUNK, or the largest chunk
+that fits in half the free memory when that is smaller.
 """
 import os
 
@@ -37,6 +38,17 @@ def max_batch(sizes, mem_bytes):
     if r < 0:
         raise ValueError(nat.load().dpnegf_last_error().decode())
     return r
+
+
+def default_chunk(systems, mem_fraction=0.5):
+    """DEFAULT_CHUNK points per blocked call, or fewer when the buffers of
+    that many would not fit in ``mem_fraction`` of the free device memory
+    (blocks of several hundred orbitals)."""
+    free, _ = torch.cuda.mem_get_info(systems.device)
+    bmax = max_batch(systems.sizes, int(free * mem_fraction))
+    if bmax < 1:
+        raise torch.cuda.OutOfMemoryError("no batch of the blocked sweep fits in free memory")
+    return min(DEFAULT_CHUNK, bmax)
 
 
 def _time_chunk(systems, s, B, trial_blocks):

@@ -24,7 +24,7 @@ import torch
 from .. import _native as nat
 from ._btd import C128, block_sizes, corner_self_energy, is_orthogonal, pack_arena, split_blocks
 from .ozaki_res_cal import ozaki_residues
-from .recursive_green_cal import _run_full
+from .recursive_green_cal import _run_full, full_points_per_call
 from .surface_green import self_energy_batched
 
 
@@ -47,15 +47,17 @@ def _sigmas(lead_L, lead_R, e, eta, E_ref, nblk0, nblkK):
     return sL, nL, sR, nR
 
 
-def _weighted_blocks(arena, B, shapes, coef):
-    """[sum_b Re(c_b X_b)] for every block of a block-major arena."""
+def _weighted_blocks(arena, B, shapes, coef, acc=None):
+    """[sum_b Re(c_b X_b)] for every block of a block-major arena, added to
+    the blocks of ``acc`` when given."""
     out = []
     off = 0
-    for (m, n) in shapes:
+    for q, (m, n) in enumerate(shapes):
         blk = arena[off * B:(off + m * n) * B]
-        res = torch.empty(m, n, dtype=torch.float64, device=arena.device)
+        res = acc[q] if acc is not None else torch.empty(m, n, dtype=torch.float64,
+                                                         device=arena.device)
         nat.call("dpnegf_weighted_re_sum", nat.stream_ptr(arena.device), nat.ptr(res),
-                 nat.ptr(blk), nat.ptr(coef), m * n, B, 0)
+                 nat.ptr(blk), nat.ptr(coef), m * n, B, int(acc is not None))
         out.append(res)
         off += m * n
     return out
@@ -86,11 +88,19 @@ def ozaki_density(blocks, lead_L, lead_R, kBT, mu_L, mu, R, poles, residues, E_r
     sL = torch.cat([s0[0], sp[0]]).contiguous()
     sR = torch.cat([s0[2], sp[2]]).contiguous()
     z = torch.cat([e0 + E_ref + 1j * eta_device0, ep + E_ref + 1j * eta_device]).contiguous()
-    raw = _run_full(z, arena, sizes, ortho, sL, s0[1], sR, s0[3], False)
     coef = torch.cat([torch.tensor([1j * R], dtype=C128, device=dev),
                       (4.0 * kBT * residues).to(C128)]).contiguous()
-    dm_eq = _weighted_blocks(raw["grd"], M + 1, [(n, n) for n in sizes], coef)
-    del raw
+    # all points in one call unless their outputs would not fit (blocks of
+    # several hundred orbitals): then in chunks, summed in order
+    dm_eq = None
+    chunk = full_points_per_call(sizes, False, M + 1, dev)
+    for c0 in range(0, M + 1, chunk):
+        c = slice(c0, min(M + 1, c0 + chunk))
+        raw = _run_full(z[c].contiguous(), arena, sizes, ortho, sL[c].contiguous(), s0[1],
+                        sR[c].contiguous(), s0[3], False)
+        dm_eq = _weighted_blocks(raw["grd"], z[c].shape[0], [(n, n) for n in sizes],
+                                 coef[c].contiguous(), dm_eq)
+        del raw
     if abs(vL - vR) <= 1e-14:
         return dm_eq, 0.
     # non-equilibrium window (density.py:183-195)
@@ -101,7 +111,6 @@ def ozaki_density(blocks, lead_L, lead_R, kBT, mu_L, mu, R, poles, residues, E_r
     G = ex.shape[0]
     sLx, nL, sRx, nR = _sigmas(lead_L, lead_R, ex, eta_lead, E_ref, sizes[0], sizes[-1])
     zx = (ex + E_ref + 1j * eta_device).contiguous()
-    raw = _run_full(zx, arena, sizes, ortho, sLx, nL, sRx, nR, True)
     nK = sizes[-1]
     gam = 1j * (sRx - sRx.mH)
     gamR = torch.zeros(G, nK, nK, dtype=C128, device=dev)
@@ -109,18 +118,28 @@ def ozaki_density(blocks, lead_L, lead_R, kBT, mu_L, mu, R, poles, residues, E_r
     xsd = xs.to(dev)
     f = fermi_R(xsd + mu) - fermi_L(xsd + mu)
     coef = (wl.to(dev) * f).to(C128).contiguous()
-    lc = split_blocks(raw["lc"], G, [(n, nK) for n in sizes])
-    dm_neq = []
-    for q, n in enumerate(sizes):
-        X = torch.empty(G, n, nK, dtype=C128, device=dev)
-        Y = torch.empty(G, n, n, dtype=C128, device=dev)
-        lcq = lc[q].contiguous()
-        st = nat.stream_ptr(dev)
-        nat.call("dpnegf_zgemm_batched", st, G, n, nK, nK, nat.ptr(lcq), nK, n * nK, 0,
-                 nat.ptr(gamR), nK, nK * nK, 0, 1.0, 0.0, 0.0, 0.0, nat.ptr(X), nK, n * nK)
-        nat.call("dpnegf_zgemm_batched", st, G, n, n, nK, nat.ptr(X), nK, n * nK, 0,
-                 nat.ptr(lcq), nK, n * nK, 1, 1.0, 0.0, 0.0, 0.0, nat.ptr(Y), n, n * n)
-        dm_neq.append(_weighted_blocks(Y.reshape(-1), G, [(n, n)], coef)[0])
+    dm_neq = [None] * len(sizes)
+    chunk = full_points_per_call(sizes, True, G, dev)
+    for c0 in range(0, G, chunk):
+        c = slice(c0, min(G, c0 + chunk))
+        Gc = zx[c].shape[0]
+        raw = _run_full(zx[c].contiguous(), arena, sizes, ortho, sLx[c].contiguous(), nL,
+                        sRx[c].contiguous(), nR, True)
+        lc = split_blocks(raw["lc"], Gc, [(n, nK) for n in sizes])
+        gR = gamR[c].contiguous()
+        cc = coef[c].contiguous()
+        for q, n in enumerate(sizes):
+            X = torch.empty(Gc, n, nK, dtype=C128, device=dev)
+            Y = torch.empty(Gc, n, n, dtype=C128, device=dev)
+            lcq = lc[q].contiguous()
+            st = nat.stream_ptr(dev)
+            nat.call("dpnegf_zgemm_batched", st, Gc, n, nK, nK, nat.ptr(lcq), nK, n * nK, 0,
+                     nat.ptr(gR), nK, nK * nK, 0, 1.0, 0.0, 0.0, 0.0, nat.ptr(X), nK, n * nK)
+            nat.call("dpnegf_zgemm_batched", st, Gc, n, n, nK, nat.ptr(X), nK, n * nK, 0,
+                     nat.ptr(lcq), nK, n * nK, 1, 1.0, 0.0, 0.0, 0.0, nat.ptr(Y), n, n * n)
+            acc = [dm_neq[q]] if dm_neq[q] is not None else None
+            dm_neq[q] = _weighted_blocks(Y.reshape(-1), Gc, [(n, n)], cc, acc)[0]
+        del raw
     return dm_eq, dm_neq
 
 

@@ -116,6 +116,27 @@ def recursive_gf(energy, hl, hd, hu, sd, su, sl, left_se, right_se, seP=None, E_
     return ans
 
 
+def full_points_per_call(sizes, need_lc, P, device, mem_fraction=0.5):
+    """Points per ``_run_full`` call (retarded outputs only) whose outputs and
+    work buffer fit in ``mem_fraction`` of the free device memory, at most
+    ``P`` and ``_MAX_B``."""
+    K = len(sizes)
+    h_bsz = nat.host_i32(list(sizes))
+    lib = nat.load()
+    w1 = lib.dpnegf_rgf_full_workspace_elems(K, h_bsz, 1)
+    w2 = lib.dpnegf_rgf_full_workspace_elems(K, h_bsz, 2)
+    nsq = sum(n * n for n in sizes)
+    nx = sum(sizes[q] * sizes[q + 1] for q in range(K - 1))
+    nlc = sum(n * sizes[-1] for n in sizes) if need_lc else 0
+    per = 16 * ((w2 - w1) + 2 * nsq + 2 * nx + nlc + sizes[0] * sizes[-1]) + 4
+    free, _ = torch.cuda.mem_get_info(device)
+    fit = (int(free * mem_fraction) - 16 * (2 * w1 - w2)) // per
+    if fit < 1:
+        raise torch.cuda.OutOfMemoryError("one point of the block sweep does not fit in free "
+                                          "device memory")
+    return max(1, min(P, fit, _MAX_B))
+
+
 def _run_full(z, arena, sizes, ortho, sigL, nL, sigR, nR, need_lc, s_in=None, s_out=None):
     """One ``dpnegf_rgf_full`` call; returns the raw block-major arenas."""
     device = z.device
@@ -186,8 +207,9 @@ def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", sta
     ``sigR`` [P, nR, nR] (already trimmed to their corners), ``sys_idx`` [P] int32
     (k-point of each point).  Returns T [P] float64 (and fills ``status``).
     ``chunk``: points per call of the block-operation sweep (blocks above 64);
-    by default ``autotune.DEFAULT_CHUNK``, or ``autotune.blocked_chunk``'s pick
-    when ``DPNEGF_AUTOTUNE=1``."""
+    by default ``autotune.default_chunk`` (DEFAULT_CHUNK, fewer when that many
+    would not fit in half the free device memory), or
+    ``autotune.blocked_chunk``'s pick when ``DPNEGF_AUTOTUNE=1``."""
     if not isinstance(systems, BTDSystems):
         raise TypeError("systems must be a BTDSystems")
     dev = systems.device
@@ -206,7 +228,8 @@ def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", sta
     K = len(systems.sizes)
     idx = (sys_idx.cpu() if sys_idx is not None else torch.zeros(P, dtype=torch.int32))
     if chunk is None:
-        chunk = autotune.blocked_chunk(systems, P) if autotune.enabled() else autotune.DEFAULT_CHUNK
+        chunk = (autotune.blocked_chunk(systems, P) if autotune.enabled()
+                 else autotune.default_chunk(systems))
     for s in range(systems.nsys):
         pts = (idx == s).nonzero().reshape(-1)
         if pts.numel() == 0:

@@ -53,6 +53,26 @@ def _orthogonal(sL, sLL):
                 and torch.count_nonzero(sLL).item() == 0)
 
 
+def points_per_call(n, a, P, device, mem_fraction=0.5):
+    """Points per ``dpnegf_lead_self_energy`` call: all ``P`` for the fused
+    kernel (leads up to 48 orbitals); for larger leads at most 65535, and no
+    more than fit, with their outputs, in ``mem_fraction`` of the free device
+    memory (the batched decimation's workspace grows by about 12 n^2 complex
+    elements per point)."""
+    if P < 1 or (n <= nat.LEAD_NMAX and a <= nat.LEAD_NMAX):
+        return max(P, 1)
+    lib = nat.load()
+    w1 = lib.dpnegf_lead_self_energy_workspace_elems(n, a, 1)
+    w2 = lib.dpnegf_lead_self_energy_workspace_elems(n, a, 2)
+    per = 16 * ((w2 - w1) + n * n + a * a) + 16 + 8 + 4 + 4
+    free, _ = torch.cuda.mem_get_info(device)
+    fit = (int(free * mem_fraction) - 16 * (2 * w1 - w2)) // per
+    if fit < 1:
+        raise torch.cuda.OutOfMemoryError(
+            f"one point of the {n}-orbital lead decimation does not fit in free device memory")
+    return min(P, fit, 65535)
+
+
 def self_energy_batched(leads, energies, eta_lead, sys_idx=None, E_ref=0.0, Bulk=False,
                         raise_on_error=True, stop=LS_STOP):
     """Self-energies of P points.
@@ -83,17 +103,23 @@ def self_energy_batched(leads, energies, eta_lead, sys_idx=None, E_ref=0.0, Bulk
     ortho = all(_orthogonal(l[3], l[4]) for l in leads)
     if len(leads) > 1 and sys_idx is None:
         raise ValueError("sys_idx is required with more than one lead system")
-    eps = torch.empty(nat.load().dpnegf_lead_self_energy_workspace_elems(n, a, P), dtype=C128,
-                      device=device)
+    if sys_idx is not None:
+        sys_idx = sys_idx.to(device=device, dtype=torch.int32).contiguous()
     gs = torch.empty(P, n, n, dtype=C128, device=device)
     sig = torch.empty(P, a, a, dtype=C128, device=device)
     iters = torch.zeros(P, dtype=torch.int32, device=device)
     status = torch.zeros(P, dtype=torch.int32, device=device)
     resid = torch.zeros(P, dtype=torch.float64, device=device)
-    nat.call("dpnegf_lead_self_energy", nat.stream_ptr(device), n, a, mode, nat.ptr(arena),
-             stride, len(leads), nat.ptr(sys_idx), P, nat.ptr(e), float(eta_lead), float(stop),
-             nat.ptr(eps), nat.ptr(gs), nat.ptr(sig), int(ortho), nat.ptr(iters),
-             nat.ptr(status), nat.ptr(resid))
+    chunk = points_per_call(n, a, P, device)
+    eps = torch.empty(nat.load().dpnegf_lead_self_energy_workspace_elems(n, a, chunk),
+                      dtype=C128, device=device)
+    for c0 in range(0, P, chunk):
+        c1 = min(P, c0 + chunk)
+        nat.call("dpnegf_lead_self_energy", nat.stream_ptr(device), n, a, mode, nat.ptr(arena),
+                 stride, len(leads), nat.ptr(sys_idx[c0:c1] if sys_idx is not None else None),
+                 c1 - c0, nat.ptr(e[c0:c1]), float(eta_lead), float(stop), nat.ptr(eps),
+                 nat.ptr(gs[c0:c1]), nat.ptr(sig[c0:c1]), int(ortho), nat.ptr(iters[c0:c1]),
+                 nat.ptr(status[c0:c1]), nat.ptr(resid[c0:c1]))
     res = SelfEnergyResult(sig, gs, iters, status, resid)
     if raise_on_error:
         check_status(res, e)

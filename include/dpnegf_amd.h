@@ -35,7 +35,7 @@ extern "C" {
 /* size limits of the entry points */
 #define DPNEGF_TRANSMISSION_NMAX 64
 #define DPNEGF_LEAD_NMAX 48
-#define DPNEGF_FULL_NMAX 256
+#define DPNEGF_FULL_NMAX 1024
 
 /* per-point status bits */
 #define DPNEGF_STATUS_SINGULAR 1

@@ -155,3 +155,73 @@ def test_entry_module_helpers_run_without_a_gpu():
     t = entry._f64([0.5, -1.0])
     assert t.dtype == torch.float64 and t.tolist() == [0.5, -1.0]
     assert callable(entry.build) and callable(entry.smoke)
+
+
+@pytest.mark.skipif(not os.path.exists(nat.lib_path()), reason="library not built")
+def test_block_limit_is_1024_orbitals():
+    """Blocks up to 1024 orbitals pass the size check (the inverse recurses
+    over diagonal halves above 128) and 1025 is refused before any device
+    call; the inverse's workspace stays 3 B n^2 at every depth."""
+    lib = nat.load()
+    assert nat.FULL_NMAX == 1024
+    rc = lib.dpnegf_rgf_transmission_blocked(None, 2, nat.host_i32([64, 1025]), None, 1, None,
+                                             None, 1, None, 1, 1, None, None, None)
+    assert rc < 0 and b"1025" in lib.dpnegf_last_error()
+    rc = lib.dpnegf_lead_self_energy(None, 1025, 1025, 0, None, 0, 1, None, 1, None, 1e-5,
+                                     1e-40, None, None, None, 1, None, None, None)
+    assert rc < 0 and b"exceed" in lib.dpnegf_last_error()
+    for n in (300, 520, 1024):
+        h = nat.host_i32([n])
+        assert lib.dpnegf_rgf_full_workspace_elems(1, h, 2) == 2 * 8 * n * n
+        assert lib.dpnegf_rgf_transmission_blocked_workspace_elems(1, h, 2) == 2 * 11 * n * n
+
+
+@pytest.mark.skipif(not os.path.exists(nat.lib_path()), reason="library not built")
+def test_points_per_call_fit_the_free_memory(monkeypatch):
+    """The chunks of the large-lead decimation, of the full-output sweep (the
+    density) and of the blocked transmission fit, with their outputs, in half
+    the free memory, and one more point would not (or the chunk is P or the
+    per-call limit)."""
+    import torch
+    from dpnegf_amd.negf import autotune
+    from dpnegf_amd.negf.recursive_green_cal import full_points_per_call
+    from dpnegf_amd.negf.surface_green import points_per_call
+    lib = nat.load()
+    free = {"v": 0}
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda device=None: (free["v"], 2 * free["v"]))
+    assert points_per_call(48, 48, 5000, None) == 5000  # fused kernel: no chunking
+    for n, a in ((300, 300), (520, 300), (1024, 1024)):
+        def need(c):
+            return 16 * (lib.dpnegf_lead_self_energy_workspace_elems(n, a, c)
+                         + c * (n * n + a * a)) + 32 * c
+        for budget in (10 ** 9, 7 * 10 ** 10, 2 * 10 ** 11):
+            free["v"] = budget
+            c = points_per_call(n, a, 4096, None)
+            assert need(c) <= budget // 2, (n, budget, c)
+            assert c in (4096, 65535) or need(c + 1) > budget // 2
+    free["v"] = 10 ** 6
+    with pytest.raises(torch.cuda.OutOfMemoryError):
+        points_per_call(1024, 1024, 10, None)
+    for sizes, lc in (([520, 300, 520], True), ([1024] * 4, False), ([64, 96], True)):
+        K = len(sizes)
+        nsq = sum(n * n for n in sizes)
+        nx = sum(sizes[q] * sizes[q + 1] for q in range(K - 1))
+        nlc = sum(n * sizes[-1] for n in sizes) if lc else 0
+
+        def need_full(c):
+            w = lib.dpnegf_rgf_full_workspace_elems(K, nat.host_i32(sizes), c)
+            return 16 * (w + c * (2 * nsq + 2 * nx + nlc + sizes[0] * sizes[-1])) + 4 * c
+        for budget in (10 ** 10, 2 * 10 ** 11):
+            free["v"] = budget
+            c = full_points_per_call(sizes, lc, 161, None)
+            assert need_full(c) <= budget // 2, (sizes, budget, c)
+            assert c == 161 or need_full(c + 1) > budget // 2
+
+    class _Sys:
+        device = None
+        sizes = [1024, 1024]
+    free["v"] = 10 ** 11
+    c = autotune.default_chunk(_Sys)
+    assert 1 <= c < autotune.DEFAULT_CHUNK
+    free["v"] = 10 ** 13
+    assert autotune.default_chunk(_Sys) == autotune.DEFAULT_CHUNK

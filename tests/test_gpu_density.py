@@ -61,9 +61,17 @@ def _oracle_density(s, kBT, mu_L, mu, R, M, E_ref, eta_lead, eta_dev, vL, vR, n_
     return dm, neq
 
 
-def test_ozaki_density_matches_oracle():
+@pytest.mark.parametrize("chunk", [0, 7])
+def test_ozaki_density_matches_oracle(chunk, monkeypatch):
+    """chunk 0: every pole (and every quadrature node) in one sweep; chunk 7:
+    the sweeps of 7 points each that a device too small for one call takes,
+    summed in order."""
+    from dpnegf_amd.negf import density
     from dpnegf_amd.negf.density import Ozaki, ozaki_density
     dev = _dev()
+    print(f"case density chunk={chunk}", flush=True)
+    if chunk:
+        monkeypatch.setattr(density, "full_points_per_call", lambda sizes, lc, P, device: chunk)
     s = wl.two_orbital_chain(5)
     kBT = 0.025852
     E_ref, mu_L, mu, R, M = -0.2, 0.05, 0.0, 1e6, 40

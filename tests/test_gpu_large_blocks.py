@@ -75,6 +75,50 @@ def test_recursive_gf_blocks_up_to_256(sizes):
     _cmp_tuple(out, ref)
 
 
+@pytest.mark.parametrize("sizes", [[300, 264], [520, 300]])
+def test_recursive_gf_blocks_above_256(sizes):
+    """Blocks above 256: the inverse's halves are themselves inverted through
+    the Schur complement (300 -> 150 -> 75, 520 -> 260 -> 130 -> 65); every
+    output block of recursive_gf, non-orthogonal S, against the oracle."""
+    test_recursive_gf_blocks_up_to_256(sizes)
+
+
+def test_transmission_and_leads_of_300_orbitals():
+    """10 orbitals per site, layers of 30 and 33 sites (blocks 300 / 330)
+    with leads of the same widths: batched Lopez-Sancho and the blocked
+    sweep above 256 orbitals, T against the oracle; the self-energies also
+    in chunks of 2 points, as a device without room for all points runs them."""
+    from dpnegf_amd.negf import surface_green as sg
+    from dpnegf_amd.negf.device_property import transmission_pipeline
+    dev = _dev()
+    s = wl.multiorbital(widths=(30, 33), K=2, norb=10, seed=12)
+    E = np.array([-0.8, 0.3, 0.6])
+    eta = 1e-4
+    print("case multiorbital 300/330", flush=True)
+    ref = []
+    for e in E:
+        sig = [osg.selfEnergy(l[0], l[1], l[3], l[4], _f64(e),
+                              hDL=l[2], sDL=l[5], etaLead=eta)[0] for l in (s.lead_L, s.lead_R)]
+        ans = orgf.recursive_gf(torch.tensor(e, dtype=torch.complex128), s.hl, s.hd, s.hu, s.sd,
+                                s.su, s.sl, sig[0], sig[1], eta=eta)
+        ref.append(float(odev.transmission(ans[0], olead.sigma2gamma(sig[0]),
+                                           olead.sigma2gamma(sig[1]))))
+    sd = s.to(dev)
+    T = transmission_pipeline(sd, _f64(E), eta_lead=eta, eta_device=eta).cpu().numpy()
+    assert np.abs(T - np.array(ref)).max() < 1e-10 * max(1.0, np.abs(ref).max()), (T, ref)
+    lead = tuple(x.to(dev) if x is not None else None for x in s.lead_R)
+    whole = sg.self_energy_batched([lead], _f64(E), eta)
+    orig = sg.points_per_call
+    try:
+        sg.points_per_call = lambda n, a, P, device: 2
+        parts = sg.self_energy_batched([lead], _f64(E), eta)
+    finally:
+        sg.points_per_call = orig
+    scale = whole.sigma.abs().max().item()
+    assert (parts.sigma - whole.sigma).abs().max().item() < 1e-13 * scale
+    assert torch.equal(parts.iters, whole.iters)
+
+
 def test_transmission_multiorbital_256_blocks():
     """8 orbitals per site, layers of 32 / 24 / 30 sites (blocks 256 / 192 /
     240): transmission through the blocked sweep with leads of 256 and 240
