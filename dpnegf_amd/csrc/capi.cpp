@@ -123,7 +123,8 @@ int dpnegf_lead_self_energy(void* stream, int n, int nred, int mode, const void*
   if (!lead || !e || !eps_ws || !gs || !sigma) return fail(DPNEGF_ERR_ARG, "null device pointer");
   if (nsys < 1) return fail(DPNEGF_ERR_ARG, "nsys must be >= 1");
   if (nsys > 1 && !sys_idx) return fail(DPNEGF_ERR_ARG, "sys_idx required when nsys > 1");
-  const int64_t need = 4 * (int64_t)n * n + 2 * (int64_t)nred * n;
+  // records: H, H01, S, S01 (n x n), then H_DL, S_DL (nred x n) in mode 0 only
+  const int64_t need = 4 * (int64_t)n * n + (mode == 0 ? 2 * (int64_t)nred * n : 0);
   if (nsys > 1 && lead_stride < need) return fail(DPNEGF_ERR_ARG, "lead_stride too small");
   negf::LeadArgs a;
   a.n = n; a.nred = nred; a.mode = mode; a.lead = (const cx<double>*)lead;

@@ -5,14 +5,18 @@ methods (``dpnegf/negf/lead_property.py:30-460``): ``self_energy`` /
 ``self_energy_cal`` compute Sigma with the batched Lopez-Sancho kernel (a 1-D
 energy tensor gives [B, n, n] in one launch instead of the joblib pool of
 ``compute_all_self_energy``, ``:599-683``), ``HDL_reduced``,
-``sigmaLR2Gamma``, ``fermi_dirac`` and ``gamma`` are unchanged.  The HDF5
-self-energy cache (``:932-984``) and Bloch unfolding (``:330-379``) are not
+``sigmaLR2Gamma``, ``fermi_dirac`` and ``gamma`` are unchanged.  Bloch-unfolded
+leads (``useBloch``, ``:330-379``) take the small cell's surface Green's
+functions at every unfolded point from one decimation launch
+(:func:`bloch_self_energy`).  The HDF5 self-energy cache (``:932-984``) is not
 part of this package.
 """
 import numpy as np
 import torch
 
+from .. import _native as nat
 from ._btd import C128
+from .bloch import Bloch
 from .surface_green import self_energy_batched
 
 _EV2J = 1.6021766208e-19
@@ -37,8 +41,13 @@ class LeadProperty(object):
         self.kpoint = None
         self.voltage_old = None
         self.useBloch = useBloch
-        if useBloch:
-            raise NotImplementedError("Bloch-unfolded leads are not supported by dpnegf_amd")
+        self.bloch_factor = bloch_factor
+        self.bloch_sorted_indice = bloch_sorted_indice
+        self.bloch_R_list = bloch_R_list
+        if self.useBloch:
+            assert self.bloch_sorted_indice is not None
+            assert self.bloch_R_list is not None
+            assert self.bloch_factor is not None
         self.device = torch.device(device)
 
     def _lead_blocks(self, kpoint):
@@ -51,13 +60,7 @@ class LeadProperty(object):
                 b.to(device=self.device, dtype=C128) for b in blocks]
             self.voltage_old = self.voltage
             self.kpoint = kp
-        subblocks = None
-        if hasattr(self.hamiltonian, "get_hs_device"):
-            try:
-                subblocks = self.hamiltonian.get_hs_device(kpoint, only_subblocks=True)
-            except TypeError:
-                subblocks = None
-        hdl, sdl = self.HDL_reduced(self.HDLk, self.SDLk, subblocks)
+        hdl, sdl = self.HDL_reduced(self.HDLk, self.SDLk, self._subblocks(kpoint))
         return (self.HLk, self.HLLk, hdl, self.SLk, self.SLLk, sdl)
 
     def self_energy(self, kpoint, energy, eta_lead=1e-5, method="Lopez-Sancho", save_path=None,
@@ -69,13 +72,41 @@ class LeadProperty(object):
 
     def self_energy_cal(self, kpoint, energy, eta_lead=1e-5, method="Lopez-Sancho",
                         se_numba_jit=None, HS_inmem=True):
-        lead = self._lead_blocks(kpoint)
         e = torch.as_tensor(energy)
-        res = self_energy_batched([lead], e.reshape(-1), eta_lead, E_ref=self.E_ref)
-        self.se = res.sigma[0] if e.ndim == 0 else res.sigma
+        if self.useBloch:
+            sig = self._bloch_self_energy(kpoint, e.reshape(-1), eta_lead)
+        else:
+            lead = self._lead_blocks(kpoint)
+            sig = self_energy_batched([lead], e.reshape(-1), eta_lead, E_ref=self.E_ref).sigma
+        self.se = sig[0] if e.ndim == 0 else sig
         if not HS_inmem:
             del self.HLk, self.HLLk, self.HDLk, self.SLk, self.SLLk, self.SDLk
         return self.se
+
+    def _subblocks(self, kpoint):
+        if hasattr(self.hamiltonian, "get_hs_device"):
+            try:
+                return self.hamiltonian.get_hs_device(kpoint, only_subblocks=True)
+            except TypeError:
+                return None
+        return None
+
+    def _bloch_self_energy(self, kpoint, e, eta_lead):
+        """``lead_property.py:330-378``: the small cell's blocks at every point
+        that folds onto ``kpoint``; the device-lead coupling is the one
+        ``get_hs_lead`` returns with them (the supercell's, at ``kpoint``)."""
+        self.kpoint = torch.tensor(kpoint)
+        self.voltage_old = self.voltage
+        kb = Bloch(self.bloch_factor).unfold_points(list(np.asarray(kpoint, dtype=float)))
+        small = []
+        for k in kb:
+            blocks = self.hamiltonian.get_hs_lead(torch.tensor(k), tab=self.tab, v=self.voltage)
+            self.HLk, self.HLLk, self.HDLk, self.SLk, self.SLLk, self.SDLk = [
+                b.to(device=self.device, dtype=C128) for b in blocks]
+            small.append((self.HLk, self.HLLk, self.SLk, self.SLLk))
+        hdl, sdl = self.HDL_reduced(self.HDLk, self.SDLk, self._subblocks(kpoint))
+        return bloch_self_energy(small, kb, self.bloch_R_list, self.bloch_sorted_indice, hdl, sdl,
+                                 e, eta_lead, self.E_ref)
 
     @staticmethod
     def HDL_reduced(HDL, SDL, subblocks):
@@ -102,3 +133,53 @@ class LeadProperty(object):
     @property
     def gamma(self):
         return self.sigmaLR2Gamma(self.se)
+
+
+def bloch_self_energy(small, k_bloch, R_list, sorted_idx, hDL, sDL, energies, eta_lead,
+                      E_ref=0.0):
+    """Self-energies [P, a, a] of a lead whose principal layer is m copies of a
+    small cell (``lead_property.py:336-378``).
+
+    ``small``: per unfolded point k' (``k_bloch`` [nk, 3]) the small cell's
+    ``(hL, hLL, sL, sLL)`` on the device; ``R_list``: the m cell offsets;
+    ``sorted_idx``: supercell orbital order in terms of kron(cell, orbital);
+    ``hDL``/``sDL`` [a, b]: device-lead coupling (rows already reduced).
+    g_s of every (E, k') comes from one decimation launch; then
+    g = mean_k' kron(P(k'), g_s(k')) with P_ij = exp(2 pi i (R_j - R_i) . k'),
+    and Sigma = (e S_DL - H_DL) g[:b, :b] (e S_DL^H - H_DL^H), e = E + E_ref
+    real-axis (no eta) as in the reference."""
+    dev = hDL.device
+    e = torch.as_tensor(energies).to(device=dev, dtype=C128).reshape(-1)
+    P, nk = e.shape[0], len(small)
+    e_all = e.repeat(nk).contiguous()  # point index k' * P + p
+    sys_idx = torch.arange(nk, dtype=torch.int32, device=dev).repeat_interleave(P).contiguous()
+    leads = [(hL, hLL, None, sL, sLL, None) for (hL, hLL, sL, sLL) in small]
+    gs = self_energy_batched(leads, e_all, eta_lead, sys_idx=sys_idx if nk > 1 else None,
+                             E_ref=E_ref).gs
+    n = gs.shape[-1]
+    m = len(R_list)
+    R = torch.stack([torch.as_tensor(r, dtype=torch.float64) for r in R_list]).to(dev)
+    kb = torch.as_tensor(np.asarray(k_bloch, dtype=float), dtype=torch.float64, device=dev)
+    # phase[k', i, j] = exp(2 pi i (R_j - R_i) . k')
+    arg = (R[None, None, :, :] - R[None, :, None, :]).mul(kb[:, None, None, :]).sum(-1)
+    phase = torch.polar(torch.ones_like(arg), 2 * np.pi * arg)
+    gs = gs.view(nk, P, n, n)
+    g = torch.zeros(P, m, n, m, n, dtype=C128, device=dev)
+    for q in range(nk):
+        g += phase[q][None, :, None, :, None] * gs[q][:, None, :, None, :]
+    g = (g / nk).reshape(P, m * n, m * n)
+    idx = torch.as_tensor(sorted_idx, device=dev)
+    b = hDL.shape[1]
+    gb = g[:, idx[:b]][:, :, idx[:b]].contiguous()
+    a = hDL.shape[0]
+    ee = (e + E_ref).view(P, 1, 1)
+    A = (ee * sDL - hDL).contiguous()                       # [P, a, b]
+    C = (ee * sDL.conj().T - hDL.conj().T).contiguous()     # [P, b, a]
+    X = torch.empty(P, a, b, dtype=C128, device=dev)
+    sig = torch.empty(P, a, a, dtype=C128, device=dev)
+    st = nat.stream_ptr(dev)
+    nat.call("dpnegf_zgemm_batched", st, P, a, b, b, nat.ptr(A), b, a * b, 0, nat.ptr(gb), b,
+             b * b, 0, 1.0, 0.0, 0.0, 0.0, nat.ptr(X), b, a * b)
+    nat.call("dpnegf_zgemm_batched", st, P, a, a, b, nat.ptr(X), b, a * b, 0, nat.ptr(C), a,
+             b * a, 0, 1.0, 0.0, 0.0, 0.0, nat.ptr(sig), a, a * a)
+    return sig

@@ -220,3 +220,67 @@ def random_btd(block_sizes, seed=0, hermitian_offdiag=True, overlap=False):
             su.append(torch.zeros(a, b, dtype=C128))
             sl.append(torch.zeros(b, a, dtype=C128))
     return hd, sd, hl, hu, sl, su
+
+
+class BlochLead:
+    """A lead whose principal layer is ``m`` copies of a small cell of ``n``
+    orbitals along a periodic transverse direction y, for Bloch-unfolded
+    self-energies (``bloch_factor = [1, m, 1]``, ``bloch_R_list`` the cell
+    offsets (0, j, 0)).  Terms, per small cell R and layer (fixed by ``seed``):
+    on-site ``h0`` (Hermitian), ``ty`` from cell R to R + 1 in y,
+    ``tx`` to cell R of the next layer, ``txy`` to cell R + 1 of the next layer.
+    The supercell is stored orbital-major (orbital a of cell i at a m + i), so
+    ``sorted_idx`` maps it onto the cell-major order of kron(P, g).
+
+    ``get_hs_lead(k)`` follows ``negf_hamiltonian_init.py:882-975``: the small
+    cell's blocks at the unfolded point ``k`` and the supercell's device-lead
+    coupling at the point ``k`` folds onto; ``supercell(k)`` gives the whole
+    supercell lead at ``k`` for a direct decimation."""
+
+    def __init__(self, m=2, n=3, seed=0, a_dev=None, overlap=False):
+        rng = np.random.default_rng(seed)
+        self.m, self.n = m, n
+        self.h0 = _herm(rng, n, 0.5)
+        self.ty = rng.normal(0, 0.3, (n, n)) + 1j * rng.normal(0, 0.1, (n, n))
+        self.tx = rng.normal(0, 0.4, (n, n))
+        self.txy = rng.normal(0, 0.15, (n, n))
+        self.s_on = np.eye(n) + (0.05 * _herm(rng, n, 1.0) if overlap else 0)
+        self.s_x = 0.03 * rng.normal(0, 1, (n, n)) if overlap else np.zeros((n, n))
+        a = a_dev if a_dev is not None else m * n
+        self.hdl = rng.normal(0, 0.3, (a, m * n)) + 1j * rng.normal(0, 0.05, (a, m * n))
+        self.sdl = 0.02 * rng.normal(0, 1, (a, m * n)) if overlap else np.zeros((a, m * n))
+        self.sorted_idx = torch.tensor([i * n + o for o in range(n) for i in range(m)])
+        self.R_list = [torch.tensor([0.0, float(j), 0.0]) for j in range(m)]
+        self.bloch_factor = [1, m, 1]
+        self.tab = "lead_L"
+
+    def small(self, k):
+        """Small-cell (hL, hLL, sL, sLL) at reduced wave vector k (3-vector),
+        H(k) = sum_d H(0 -> d) exp(-2 pi i k d): the sign that the unfolding's
+        phase exp(2 pi i (R_j - R_i) . k) assumes."""
+        ph = np.exp(-2j * np.pi * float(k[1]))
+        hL = self.h0 + self.ty * ph + self.ty.conj().T / ph
+        hLL = self.tx + self.txy * ph
+        return hL, hLL, self.s_on.astype(complex), self.s_x.astype(complex)
+
+    def supercell(self, k):
+        """Supercell (hL, hLL, sL, sLL) at k, orbital-major."""
+        m, n = self.m, self.n
+        ph = np.exp(-2j * np.pi * float(k[1]))
+        big = [np.zeros((m * n, m * n), complex) for _ in range(4)]
+        terms = [(0, 0, self.h0), (0, 1, self.ty), (0, -1, self.ty.conj().T),  # in-layer
+                 (1, 0, self.tx), (1, 1, self.txy),  # to the next layer
+                 (2, 0, self.s_on), (3, 0, self.s_x)]
+        for which, d, t in terms:
+            for i in range(m):
+                j, L = (i + d) % m, (i + d) // m
+                blk = t * ph ** L
+                for o1 in range(n):
+                    for o2 in range(n):
+                        big[which][o1 * m + i, o2 * m + j] += blk[o1, o2]
+        return tuple(big)
+
+    def get_hs_lead(self, kpoint, tab=None, v=0.0):
+        k = np.asarray(kpoint, dtype=float).reshape(3)
+        hL, hLL, sL, sLL = self.small(k)
+        return (_t(hL - v * sL), _t(hLL - v * sLL), _t(self.hdl), _t(sL), _t(sLL), _t(self.sdl))

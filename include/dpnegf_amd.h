@@ -72,8 +72,9 @@ int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h
  * cores (surface_green.py:102-159, 169-226), i.e. LeadProperty.self_energy_cal
  * (dpnegf/negf/lead_property.py:266-384) over a batch of (E, k).
  *   lead (per system, at lead + sys * lead_stride): hL, hLL, sL, sLL [n x n],
- *        hDL, sDL [nred x n]   (the tuple of get_hs_lead,
- *        dpnegf/negf/negf_hamiltonian_init.py:974, with HDL_reduced applied)
+ *        then in mode 0 hDL, sDL [nred x n]   (the tuple of get_hs_lead,
+ *        dpnegf/negf/negf_hamiltonian_init.py:974, with HDL_reduced applied);
+ *        lead_stride >= 4 n^2 (+ 2 nred n in mode 0) when nsys > 1
  *   e[p]   = E + E_ref (complex allowed); the decimation uses e + i*eta
  *   mode 0: Sigma = (e sDL - hDL) g_s (e sDL^H - hDL^H)      [nred x nred]
  *   mode 1: Sigma = (e sL - hL) - g_s^{-1};  mode 2 (Bulk): g_s^{-1}

@@ -225,3 +225,15 @@ def test_points_per_call_fit_the_free_memory(monkeypatch):
     assert 1 <= c < autotune.DEFAULT_CHUNK
     free["v"] = 10 ** 13
     assert autotune.default_chunk(_Sys) == autotune.DEFAULT_CHUNK
+
+
+@pytest.mark.skipif(not os.path.exists(nat.lib_path()), reason="library not built")
+def test_lead_records_need_coupling_blocks_only_in_mode_0():
+    """With several lead systems, a record of 4 n^2 elements (no H_DL, S_DL)
+    is too short when Sigma needs the coupling (mode 0); the stride check
+    refuses it before any device work."""
+    lib = nat.load()
+    fake = ctypes.c_void_p(16)  # never dereferenced: the call fails at the stride check
+    rc = lib.dpnegf_lead_self_energy(None, 4, 4, 0, fake, 4 * 16, 2, fake, 1, fake, 1e-5,
+                                     1e-40, fake, fake, fake, 1, None, None, None)
+    assert rc < 0 and b"lead_stride" in lib.dpnegf_last_error()

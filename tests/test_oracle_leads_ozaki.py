@@ -158,3 +158,32 @@ def test_gauss_xw_integrates_polynomials():
     for p in range(0, 19):
         exact = (1.7 ** (p + 1) - (-0.3) ** (p + 1)) / (p + 1)
         assert abs(float((w * xs ** p).sum()) - exact) < 1e-12 * max(1, abs(exact))
+
+
+@pytest.mark.parametrize("m,K,overlap", [(2, 0.0, False), (3, 0.3, True), (4, 0.71, False)])
+def test_bloch_unfolded_self_energy_equals_the_supercell(m, K, overlap):
+    """Bloch's theorem pins the restated useBloch branch
+    (``lead_property.py:336-378``): for a lead whose layer is m copies of a
+    small cell, the average of kron(P(k'), g_s(k')) over the m unfolded points
+    k' = (K + j)/m is the supercell's own surface Green's function at K, so its
+    self-energy equals the direct decimation of the supercell.  The supercell is
+    stored orbital-major, which also exercises ``bloch_sorted_indice``."""
+    lead = wl.BlochLead(m=m, n=3, seed=10 + m, overlap=overlap, a_dev=5)
+    kpt = [0.0, K, 0.0]
+    kb = olead.bloch_unfold(lead.bloch_factor, kpt)
+    assert kb.shape == (m, 3) and np.allclose(kb[:, 1], (K + np.arange(m)) / m)
+    eta = 1e-4
+    big = lead.supercell(kpt)
+    for E in (-0.7, 0.15, 0.9):
+        e64 = torch.tensor(E, dtype=torch.float64)
+        gs = [osg.selfEnergy(*lead.small(k), e64, etaLead=eta)[1] for k in kb]
+        sig = olead.bloch_self_energy(gs, kb, lead.R_list, lead.sorted_idx,
+                                      torch.as_tensor(lead.hdl), torch.as_tensor(lead.sdl), E)
+        ref = osg.selfEnergy(*big, e64, hDL=lead.hdl, sDL=lead.sdl, etaLead=eta)[0]
+        err = (sig - ref).abs().max().item() / ref.abs().max().item()
+        assert err < 1e-10, (E, err)
+        # a plausible slip, the phase with R_i - R_j, breaks the identity
+        bad = olead.bloch_self_energy(gs, -kb, lead.R_list, lead.sorted_idx,
+                                      torch.as_tensor(lead.hdl), torch.as_tensor(lead.sdl), E)
+        if K != 0.0 or m > 2:
+            assert (bad - ref).abs().max().item() > 1e-4 * ref.abs().max().item()
