@@ -2,12 +2,14 @@
 
 * :class:`DeviceProperty` -- ``cal_green_function`` (``device_property.py:137-296``),
   ``_cal_tc_`` (``:374-406``), ``_cal_dos_`` (``:408-439``), ``_cal_ldos_``
-  (``:441-470``) and the Green's-function properties, backed by
-  :func:`recursive_gf`.
+  (``:441-470``), ``_cal_local_current_`` (``:472-503``, every atom pair at
+  once), ``_cal_current_nscf_`` / ``_cal_current_`` (``:308-372``) and the
+  Green's-function properties, backed by :func:`recursive_gf`.
 * :func:`transmission_pipeline` -- new: T(E, k) for a whole energy grid and
   k-mesh in two launches (all lead self-energies, then the fused sweep),
   replacing the runner's chunk loop (``runner/NEGF.py:750-816``) for T.
 """
+import logging
 import math
 import os
 
@@ -18,6 +20,8 @@ from .. import _native as nat
 from ._btd import C128
 from .recursive_green_cal import BTDSystems, recursive_gf, rgf_transmission
 from .surface_green import self_energy_batched
+
+log = logging.getLogger(__name__)
 
 _EV2J = 1.6021766208e-19
 _BOLTZMANN = 1.380649e-23
@@ -77,6 +81,7 @@ class DeviceProperty(object):
         assert energy.ndim == 1
         B = energy.shape[0]
         batched = B > 1
+        self.ee = energy
         self.block_tridiagonal = block_tridiagonal
         kp = torch.tensor(kpoint)
         if self.kpoint is None or abs(self.kpoint - kp).sum() > 1e-5:
@@ -202,6 +207,84 @@ class DeviceProperty(object):
         ldos = torch.stack([ldos[..., acc[i]:acc[i + 1]].sum(-1) for i in range(len(acc) - 1)],
                            dim=-1)
         return ldos * 2
+
+    def _cal_local_current_(self):
+        """``device_property.py:472-503``: bond currents between atoms from the
+        whole-device Hamiltonian (no block-tridiagonal split) and the lesser
+        G^n, J_ij = Re Tr[H_ij (i G^n_ji)] - Re Tr[H_ji (i G^n_ij)].  The
+        reference loops over atom pairs; here the elementwise products are
+        summed per atom pair by two indicator products.  A batch of energies
+        gives [B, na, na]."""
+        na = len(self.norbs_per_atom)
+        hd = self.hamiltonian.get_hs_device(kpoint=self.kpoint, V=self.V,
+                                            block_tridiagonal=self.block_tridiagonal)[0][0]
+        hd = hd.to(device=self.rgf_device, dtype=C128)
+        gn = self.gnd[0]
+        e_ab = (hd * (1j * gn.transpose(-2, -1))).real  # Re H[a, b] (i G^n[b, a])
+        atom = torch.repeat_interleave(torch.arange(na, device=self.rgf_device),
+                                       torch.as_tensor(list(self.norbs_per_atom),
+                                                       device=self.rgf_device))
+        ind = torch.nn.functional.one_hot(atom, na).to(torch.float64)
+        S = ind.T @ e_ab @ ind
+        return (S - S.transpose(-2, -1)).contiguous()
+
+    def _cal_current_nscf_(self, energy_grid, tc):
+        """``device_property.py:333-372``: I-V without self-consistency from the
+        zero-bias T(E) on ``energy_grid`` (relative to the Fermi level): for
+        bias vv = 2 dv on a 0.1 V grid inside the window the energy grid
+        allows, Simpson's rule over (f(E + E_F; E_F - vm + dv) -
+        f(E + E_F; E_F - vm - dv)) T(E).  Returns (vv, [I])."""
+        from scipy.integrate import simpson
+        if abs(self.lead_L.efermi - self.lead_R.efermi) < 5e-4:
+            log.warning("The Fermi energy of the left and right leads should be equal in nscf "
+                        "current calculation.")
+        efermi = self.lead_L.efermi
+        eg = torch.as_tensor(energy_grid).detach().cpu().to(torch.float64)
+        tc = torch.as_tensor(tc).detach().cpu().to(torch.float64)
+        f = lambda x, mu: 1 / (1 + torch.exp((x - mu) / self.kBT))
+        vmin = eg.min() + 4 * self.kBT
+        vmax = eg.max() - 4 * self.kBT
+        vm = 0.5 * (vmin + vmax)
+        vmid = vm - vmin
+        vv = torch.linspace(start=0., end=float(vmid), steps=int(vmid / 0.1) + 1) * 2
+        cc = []
+        for dv in vv * 0.5:
+            y = (f(eg + efermi, efermi - vm + dv) - f(eg + efermi, efermi - vm - dv)) * tc
+            cc.append(simpson(y=y.numpy(), x=eg.numpy()))
+        return vv, cc
+
+    def _cal_current_(self, espacing=None):
+        """``device_property.py:308-331``: Simpson's rule over the energies of
+        the last ``cal_green_function`` call of (f_L - f_R) T.  The reference
+        stores it without returning it (and never sets its grid); here the grid
+        is that call's real energies and the value is also returned."""
+        from scipy.integrate import simpson
+        ee = self.ee.real.detach().cpu().to(torch.float64)
+        fl = self.lead_L.fermi_dirac(ee + self.E_ref)
+        fr = self.lead_R.fermi_dirac(ee + self.E_ref)
+        tc = torch.as_tensor(self.tc).detach().cpu().reshape(-1).to(torch.float64)
+        self.__CURRENT__ = simpson(y=((fl - fr) * tc).numpy(), x=ee.numpy())
+        return self.__CURRENT__
+
+    @property
+    def current(self):
+        return self._cal_current_()
+
+    @property
+    def lcurrent(self):
+        return self._cal_local_current_()
+
+    @property
+    def positions(self):
+        return self.structure.positions
+
+    def get_index(self, iatom):
+        """[start, end) of atom ``iatom``'s orbitals (``device_property.py:613-629``)."""
+        start = sum(self.norbs_per_atom[:iatom])
+        return [start, start + self.norbs_per_atom[iatom]]
+
+    def get_index_block(self, iatom):
+        pass
 
     @property
     def dos(self):

@@ -4,6 +4,8 @@
 * ``build_s_in``    -> ``_build_s_in_batched`` ``:21-31`` and ``:259-272``
 * ``transmission``  -> ``_cal_tc_`` ``:374-406``
 * ``dos`` / ``ldos``-> ``_cal_dos_`` ``:408-439`` / ``_cal_ldos_`` ``:441-470``
+* ``local_current`` -> ``_cal_local_current_`` ``:472-503``
+* ``current_nscf``  -> ``_cal_current_nscf_`` ``:333-372``
 """
 import math
 
@@ -74,3 +76,38 @@ def build_s_in(hd_shapes, seinL, seinR):
     s_in[0][:, :r0, :r0] += seinL[:, :r0, :r0]
     s_in[-1][:, -r1:, -r1:] += seinR[:, -r1:, -r1:]
     return s_in
+
+
+def local_current(hd, gnd, norbs_per_atom):
+    """Bond currents between atoms from the whole-device H and lesser G^n
+    (``device_property.py:472-503``), atom by atom as the reference loops:
+    J_ij = Re Tr[H_ij (i G^n_ji)] - Re Tr[H_ji (i G^n_ij)], J_ii = 0."""
+    na = len(norbs_per_atom)
+    acc = np.cumsum([0] + list(norbs_per_atom))
+    J = torch.zeros(na, na, dtype=torch.float64)
+    for i in range(na):
+        for j in range(na):
+            if i != j:
+                a0, a1, b0, b1 = acc[i], acc[i + 1], acc[j], acc[j + 1]
+                ki = hd[a0:a1, b0:b1] @ (1j * gnd[b0:b1, a0:a1])
+                kj = hd[b0:b1, a0:a1] @ (1j * gnd[a0:a1, b0:b1])
+                J[i, j] = ki.real.diag().sum() - kj.real.diag().sum()
+    return J
+
+
+def current_nscf(energy_grid, tc, efermi, kbt):
+    """Non-self-consistent I-V from the zero-bias T(E) (``:333-372``): biases
+    vv = 2 dv on a 0.1 V step up to the window the grid allows, each current
+    Simpson's rule over (f(E + E_F; E_F - vm + dv) - f(E + E_F; E_F - vm - dv)) T."""
+    from scipy.integrate import simpson
+    f = lambda x, mu: 1 / (1 + torch.exp((x - mu) / kbt))
+    emin, emax = energy_grid.min(), energy_grid.max()
+    vmin, vmax = emin + 4 * kbt, emax - 4 * kbt
+    vm = 0.5 * (vmin + vmax)
+    vmid = vm - vmin
+    vv = torch.linspace(start=0., end=float(vmid), steps=int(vmid / 0.1) + 1) * 2
+    cc = []
+    for dv in vv * 0.5:
+        y = (f(energy_grid + efermi, efermi - vm + dv) - f(energy_grid + efermi, efermi - vm - dv)) * tc
+        cc.append(simpson(y=y.numpy(), x=energy_grid.numpy()))
+    return vv, cc
