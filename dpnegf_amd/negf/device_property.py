@@ -266,6 +266,15 @@ class DeviceProperty(object):
         self.__CURRENT__ = simpson(y=((fl - fr) * tc).numpy(), x=ee.numpy())
         return self.__CURRENT__
 
+    def _cal_density_(self, dm_options):
+        """``device_property.py:505-529``: the Ozaki density of the current
+        k-point, ``dm_options`` being ``Ozaki``'s constructor arguments.  The
+        reference hands ``rgf_device`` to ``integrate`` where the device
+        object belongs; this passes the device object."""
+        from .density import Ozaki
+        dm = Ozaki(**dm_options)
+        return dm.integrate(deviceprop=self, kpoint=self.kpoint)
+
     @property
     def current(self):
         return self._cal_current_()
