@@ -139,3 +139,44 @@ def test_checkpoint_auto_keeps_every_block_when_it_fits():
     sizes = [8] * 9
     assert checkpoint_stride(None, sizes, 4, dev) == 0
     assert checkpoint_stride(None, sizes, 4, dev, mem_fraction=0.0) == 3
+
+
+@pytest.mark.parametrize("sizes", [[140, 96, 130], [300, 264]])
+def test_adjoint_blocks_above_128_matches_autograd(sizes):
+    """Blocks above 128 (the Schur-complement inverse, recursive above 256)
+    in the adjoint sweep: gradients of sum_b w_b T(E_b) against autograd
+    through the oracle, full storage and checkpointed."""
+    from dpnegf_amd.negf.adjoint import transmission
+    from tests.test_gpu_rgf import _rand_se
+    dev = _dev()
+    print(f"case adjoint sizes={sizes}", flush=True)
+    hd, sd, hl, hu, sl, su = wl.random_btd(sizes, seed=6, overlap=True)
+    B, eta = 2, 1e-3
+    sL, sR = _rand_se(B, sizes[0], 4), _rand_se(B, sizes[-1], 5)
+    E = [-0.4, 0.35]
+    w = torch.tensor([0.9, -0.6], dtype=torch.float64)
+    p_hd = [h.clone().requires_grad_(True) for h in hd]
+    p_hu = [h.clone().requires_grad_(True) for h in hu]
+    p_hl = [h.clone().requires_grad_(True) for h in hl]
+    J = 0
+    for b, e in enumerate(E):
+        z = torch.tensor(e, dtype=torch.complex128)
+        ans = orgf.recursive_gf(z, p_hl, p_hd, p_hu, sd, su, sl, sL[b], sR[b], eta=eta,
+                                keep_gr_left=False)
+        J = J + w[b] * odev.transmission(ans[0], olead.sigma2gamma(sL[b]),
+                                         olead.sigma2gamma(sR[b]))
+    ref = torch.autograd.grad(J, p_hd + p_hu + p_hl)
+    scale = max(r.abs().max().item() for r in ref)
+    to = lambda xs: [x.to(dev) for x in xs]
+    for ck in (0, 1):
+        params = [x.to(dev).requires_grad_(True) for x in hd + hu + hl]
+        K = len(sizes)
+        T = transmission(_f64(E), params[:K], params[K:2 * K - 1], params[2 * K - 1:], to(sd),
+                         to(su), to(sl), sL.to(dev), sR.to(dev), eta, checkpoint=ck)
+        Jg = (T * w.to(dev)).sum()
+        Jv = float(J.detach())
+        assert abs(float(Jg.detach()) - Jv) < 1e-10 * max(1.0, abs(Jv))
+        got = torch.autograd.grad(Jg, params)
+        for r, g in zip(ref, got):
+            err = (r - g.cpu()).abs().max().item() / scale
+            assert err < 1e-9, (ck, err)
