@@ -51,7 +51,7 @@ class LeadProperty(object):
         self.device = torch.device(device)
 
     def _lead_blocks(self, kpoint):
-        kp = torch.tensor(kpoint)
+        kp = torch.as_tensor(kpoint).detach().cpu().to(torch.float64)
         if (not hasattr(self, "HLk")) or self.voltage_old is None \
                 or abs(self.voltage_old - self.voltage) > 1e-6 \
                 or max(abs(self.kpoint - kp)) > 1e-6:
@@ -95,9 +95,9 @@ class LeadProperty(object):
         """``lead_property.py:330-378``: the small cell's blocks at every point
         that folds onto ``kpoint``; the device-lead coupling is the one
         ``get_hs_lead`` returns with them (the supercell's, at ``kpoint``)."""
-        self.kpoint = torch.tensor(kpoint)
+        self.kpoint = torch.as_tensor(kpoint).detach().cpu().to(torch.float64)
         self.voltage_old = self.voltage
-        kb = Bloch(self.bloch_factor).unfold_points(list(np.asarray(kpoint, dtype=float)))
+        kb = Bloch(self.bloch_factor).unfold_points(list(self.kpoint.numpy()))
         small = []
         for k in kb:
             blocks = self.hamiltonian.get_hs_lead(torch.tensor(k), tab=self.tab, v=self.voltage)

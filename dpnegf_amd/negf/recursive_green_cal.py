@@ -160,7 +160,7 @@ def _run_full(z, arena, sizes, ortho, sigL, nL, sigR, nR, need_lc, s_in=None, s_
     out["greater"] = ((s_out, alloc(nsq), alloc(nsq), alloc(nx), alloc(nx))
                       if s_out is not None else None)
     status = torch.zeros(B, dtype=torch.int32, device=device)
-    # the library's own size: it includes the two-level inverse's workspace for blocks above 128
+    # the library's own size: it includes the block inverse's workspace for blocks above 128
     nwork = nat.load().dpnegf_rgf_full_workspace_elems(K, nat.host_i32(sizes), B)
     if nwork < 0:
         raise ValueError(nat.load().dpnegf_last_error().decode())

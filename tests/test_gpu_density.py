@@ -94,3 +94,60 @@ def test_ozaki_density_matches_oracle(chunk, monkeypatch):
     for q in range(len(neq)):
         err = (neq[q].cpu() - torch.as_tensor(ref_neq[q])).abs().max().item() / scale
         assert err < 1e-10, (q, err)
+
+
+class _Ham:
+    """Device and lead blocks of a BTDSystem behind the reference's
+    Hamiltonian-object calls (``get_hs_device``, ``get_hs_lead`` with the
+    lead's voltage shift H - v S, ``device_norbs``)."""
+
+    def __init__(self, s):
+        self.s, self.device_norbs = s, [1] * sum(s.sizes)
+
+    def get_hs_device(self, kpoint, V=None, block_tridiagonal=True, only_subblocks=False):
+        if only_subblocks:
+            return None
+        s = self.s
+        return [list(s.hd), list(s.sd), list(s.hl), list(s.su), list(s.sl), list(s.hu)]
+
+    def get_hs_lead(self, kpoint, tab, v):
+        hL, hLL, hDL, sL, sLL, sDL = self.s.lead_L if tab == "lead_L" else self.s.lead_R
+        return hL - v * sL, hLL - v * sLL, hDL, sL, sLL, sDL
+
+
+def test_ozaki_integrate_through_device_and_lead_objects():
+    """``Ozaki.integrate`` and ``DeviceProperty._cal_density_`` as a user of
+    the reference calls them (``density.py:143-199``, ``device_property.py:505-529``):
+    LeadProperty leads at +-0.05 V around a Fermi level of -0.2 eV (also the
+    reference energy), against the oracle density with the same shifted leads."""
+    import dataclasses
+    from dpnegf_amd.negf.density import Ozaki
+    from dpnegf_amd.negf.device_property import DeviceProperty
+    from dpnegf_amd.negf.lead_property import LeadProperty
+    dev = _dev()
+    s = wl.two_orbital_chain(5)
+    ef, vL, vR, R, M, n_gauss = -0.2, 0.05, -0.05, 1e6, 40, 8
+    ham = _Ham(s)
+    leads = [LeadProperty(tab, ham, None, None, v, e_T=300, efermi=ef, device=dev)
+             for tab, v in (("lead_L", vL), ("lead_R", vR))]
+    dp = DeviceProperty(ham, None, None, e_T=300, efermi=ef, rgf_device=dev)
+    dp.set_leadLR(*leads)
+    dp.kpoint = torch.tensor([0.0, 0.0, 0.0])
+    kBT = dp.kBT
+    mu_L, mu = ef - vL, ef - 0.5 * (vL + vR)
+    fL = lambda x: 1 / (1 + torch.exp((x - (ef - vL)) / kBT))
+    fR = lambda x: 1 / (1 + torch.exp((x - (ef - vR)) / kBT))
+    shifted = dataclasses.replace(s, lead_L=ham.get_hs_lead(None, "lead_L", vL),
+                                  lead_R=ham.get_hs_lead(None, "lead_R", vR))
+    ref_eq, ref_neq = _oracle_density(shifted, kBT, mu_L, mu, R, M, ef, 1e-5, 0.0, vL, vR,
+                                      n_gauss, fL, fR)
+    eq, neq = dp._cal_density_({"R": R, "M_cut": M, "n_gauss": n_gauss})
+    eq2, neq2 = Ozaki(R, M, n_gauss).integrate(dp, [0, 0, 0], all_blocks=True)
+    assert torch.equal(eq, eq2[0]) and torch.equal(neq, neq2[0])
+    scale = ref_eq[0].abs().max().item()
+    assert (eq.cpu() - ref_eq[0].reshape(eq.shape)).abs().max().item() < 1e-10 * scale
+    for q in range(len(eq2)):
+        assert (eq2[q].cpu() - ref_eq[q].reshape(eq2[q].shape)).abs().max().item() < 1e-10 * scale
+    scale = max(float(np.abs(np.asarray(r)).max()) for r in ref_neq)
+    for q in range(len(neq2)):
+        assert (neq2[q].cpu() - torch.as_tensor(ref_neq[q])).abs().max().item() < 1e-10 * scale
