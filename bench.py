@@ -328,8 +328,8 @@ def main():
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box in this setup has a single MI355X and "
-              "the (E,k) points shard trivially across GPUs (see DESIGN.md); run with --gpus 1.",
+        print("bench.py measures one GPU only: the (E,k) points are independent, so N GPUs "
+              "would each sweep 1/N of them with no communication; run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "multiorbital":
