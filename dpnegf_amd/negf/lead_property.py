@@ -23,6 +23,14 @@ _EV2J = 1.6021766208e-19
 _BOLTZMANN = 1.380649e-23
 
 
+def _kvec(kpoint):
+    """A k-point as a host float64 tensor (a list goes through numpy, since
+    torch would make float32 of Python floats)."""
+    if torch.is_tensor(kpoint):
+        return kpoint.detach().cpu().to(torch.float64)
+    return torch.as_tensor(np.asarray(kpoint, dtype=float))
+
+
 class LeadProperty(object):
     def __init__(self, tab, hamiltonian, structure, results_path, voltage,
                  structure_leads_fold=None, bloch_sorted_indice=None, useBloch=False,
@@ -51,7 +59,7 @@ class LeadProperty(object):
         self.device = torch.device(device)
 
     def _lead_blocks(self, kpoint):
-        kp = torch.as_tensor(kpoint).detach().cpu().to(torch.float64)
+        kp = _kvec(kpoint)
         if (not hasattr(self, "HLk")) or self.voltage_old is None \
                 or abs(self.voltage_old - self.voltage) > 1e-6 \
                 or max(abs(self.kpoint - kp)) > 1e-6:
@@ -95,7 +103,7 @@ class LeadProperty(object):
         """``lead_property.py:330-378``: the small cell's blocks at every point
         that folds onto ``kpoint``; the device-lead coupling is the one
         ``get_hs_lead`` returns with them (the supercell's, at ``kpoint``)."""
-        self.kpoint = torch.as_tensor(kpoint).detach().cpu().to(torch.float64)
+        self.kpoint = _kvec(kpoint)
         self.voltage_old = self.voltage
         kb = Bloch(self.bloch_factor).unfold_points(list(self.kpoint.numpy()))
         small = []

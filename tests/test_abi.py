@@ -237,3 +237,13 @@ def test_lead_records_need_coupling_blocks_only_in_mode_0():
     rc = lib.dpnegf_lead_self_energy(None, 4, 4, 0, fake, 4 * 16, 2, fake, 1, fake, 1e-5,
                                      1e-40, fake, fake, fake, 1, None, None, None)
     assert rc < 0 and b"lead_stride" in lib.dpnegf_last_error()
+
+
+def test_lead_kpoints_stay_float64():
+    """Regression: a k-point list went through torch's default float32, which
+    moved the unfolded Bloch points by ~1e-8 and the self-energy by ~1e-7."""
+    import torch
+    from dpnegf_amd.negf.lead_property import _kvec
+    assert _kvec([0, 0.37, 0]).dtype == torch.float64 and _kvec([0, 0.37, 0])[1].item() == 0.37
+    t = torch.tensor([0.1, 0.37, 0.0], dtype=torch.float64)
+    assert torch.equal(_kvec(t), t)
