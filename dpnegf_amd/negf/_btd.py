@@ -5,6 +5,7 @@ in the order hd_q, sd_q, hu_q, su_q, hl_q, sl_q (q = 0..K-1, the last block
 without couplings).  These helpers build it with torch ops on the GPU; they
 carry no NEGF arithmetic.
 """
+import numpy as np
 import torch
 
 C128 = torch.complex128
@@ -77,3 +78,11 @@ def corner_self_energy(se, n_block, B, at_end, device):
         raise ValueError("self-energy corner must be square")
     se = se[:, -r:, -c:] if at_end else se[:, :r, :c]
     return se.to(device=device, dtype=C128).contiguous(), r
+
+
+def kvec(kpoint):
+    """A k-point as a host float64 tensor (a list goes through numpy, since
+    torch would make float32 of Python floats under its default dtype)."""
+    if torch.is_tensor(kpoint):
+        return kpoint.detach().cpu().to(torch.float64)
+    return torch.as_tensor(np.asarray(kpoint, dtype=float))

@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from .. import _native as nat
-from ._btd import C128
+from ._btd import C128, kvec
 from .recursive_green_cal import BTDSystems, recursive_gf, rgf_transmission
 from .surface_green import self_energy_batched
 
@@ -83,7 +83,7 @@ class DeviceProperty(object):
         batched = B > 1
         self.ee = energy
         self.block_tridiagonal = block_tridiagonal
-        kp = torch.tensor(kpoint)
+        kp = kvec(kpoint)
         if self.kpoint is None or abs(self.kpoint - kp).sum() > 1e-5:
             self.kpoint = kp
             self.newK_flag = True

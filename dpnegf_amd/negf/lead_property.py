@@ -15,20 +15,12 @@ import numpy as np
 import torch
 
 from .. import _native as nat
-from ._btd import C128
+from ._btd import C128, kvec as _kvec
 from .bloch import Bloch
 from .surface_green import self_energy_batched
 
 _EV2J = 1.6021766208e-19
 _BOLTZMANN = 1.380649e-23
-
-
-def _kvec(kpoint):
-    """A k-point as a host float64 tensor (a list goes through numpy, since
-    torch would make float32 of Python floats)."""
-    if torch.is_tensor(kpoint):
-        return kpoint.detach().cpu().to(torch.float64)
-    return torch.as_tensor(np.asarray(kpoint, dtype=float))
 
 
 class LeadProperty(object):
