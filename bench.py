@@ -31,6 +31,7 @@ checkpointing.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
                        [--workload ribbon48|multiorbital|density|adjoint] [--kpoints NK]
+                       [--norb N --widths W1,W2,.. --layers K --mo-energies NE]  (multiorbital)
 """
 import argparse
 import json
@@ -122,16 +123,20 @@ def cpu_baseline(s, E, eta, n_sample):
 
 
 def bench_multiorbital(args):
-    """Blocks 96/120/144/168, 40 layers, args.kpoints k-points x 1024 energies."""
+    """By default blocks 96/120/144/168 (3 orbitals/site, widths 32/40/48/56),
+    40 layers, args.kpoints k-points x 1024 energies; --norb, --widths,
+    --layers and --mo-energies change the device (8 orbitals/site on widths
+    64/60 gives blocks of 512 and 480)."""
     from dpnegf_amd import workloads as wl
     from dpnegf_amd.negf.recursive_green_cal import BTDSystems, rgf_transmission
     from dpnegf_amd.negf.surface_green import self_energy_batched
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    nk, nE, eta = args.kpoints, 1024, 1e-5
-    systems = [wl.multiorbital(widths=(32, 40, 48, 56), K=40, norb=3, kpoint=k / 32, seed=0).to(dev)
-               for k in range(nk)]
+    nk, nE, eta = args.kpoints, args.mo_energies, 1e-5
+    widths = tuple(int(w) for w in args.widths.split(","))
+    systems = [wl.multiorbital(widths=widths, K=args.layers, norb=args.norb, kpoint=k / 32,
+                               seed=0).to(dev) for k in range(nk)]
     E = torch.linspace(-2.0, 2.0, nE, dtype=torch.float64, device=dev)
     e_all = E.repeat(nk).to(torch.complex128).contiguous()
     sys_idx = torch.arange(nk, dtype=torch.int32, device=dev).repeat_interleave(nE).contiguous()
@@ -183,8 +188,10 @@ def bench_multiorbital(args):
         "ms_per_step": round(1e3 * dt / args.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"multiorbital 3 orb/site, widths 32/40/48/56 (blocks 96-168), "
-                               f"40 layers, seed 0, {nk} k-points x {nE} energies in [-2,2] eV, "
+        "config": {"workload": f"multiorbital {args.norb} orb/site, widths "
+                               f"{'/'.join(map(str, widths))} (blocks {min(bs.sizes)}-"
+                               f"{max(bs.sizes)}), {args.layers} layers, seed 0, {nk} k-points x "
+                               f"{nE} energies in [-2,2] eV, "
                                f"eta=1e-5; step = Sigma_L + Sigma_R (Lopez-Sancho) + block-"
                                f"operation RGF transmission", "points_per_step": P},
         "lead_L_ms": round(leg[0], 3), "lead_R_ms": round(leg[1], 3), "rgf_ms": round(leg[2], 3),
@@ -326,6 +333,11 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--norb", type=int, default=3, help="multiorbital: orbitals per site")
+    ap.add_argument("--widths", default="32,40,48,56", help="multiorbital: layer widths (sites)")
+    ap.add_argument("--layers", type=int, default=40, help="multiorbital: number of layers")
+    ap.add_argument("--mo-energies", type=int, default=1024,
+                    help="multiorbital: energies per k-point")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the (E,k) points are independent, so N GPUs "
