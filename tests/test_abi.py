@@ -247,3 +247,14 @@ def test_lead_kpoints_stay_float64():
     assert _kvec([0, 0.37, 0]).dtype == torch.float64 and _kvec([0, 0.37, 0])[1].item() == 0.37
     t = torch.tensor([0.1, 0.37, 0.0], dtype=torch.float64)
     assert torch.equal(_kvec(t), t)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with status 2 and says why,
+    before touching a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
