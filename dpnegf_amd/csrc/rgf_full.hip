@@ -585,8 +585,11 @@ static hipError_t gemm(hipStream_t st, int B, int m, int n, int k, Opnd A, Opnd 
   a.m = m; a.n = n; a.k = k; a.B = B;
   a.A = A; a.B_ = Bo; a.alpha = alpha; a.beta = beta;
   a.C = C; a.Cin = Cin ? Cin : C; a.cstride = cstride; a.ldc = ldc; a.z = z;
-  // 32-wide tiles in a dimension whose last 64-wide tile would be at most half full
-  const bool fr1 = m % 64 != 0 && m % 64 <= 32, fc1 = n % 64 != 0 && n % 64 <= 32;
+  // 32-wide tiles in a dimension below 256 whose last 64-wide tile would be at
+  // most half full (there the padding is a large share of the work; from 256 on
+  // it is at most 11%, and 64-wide tiles run faster than that)
+  const bool fr1 = m < 256 && m % 64 != 0 && m % 64 <= 32;
+  const bool fc1 = n < 256 && n % 64 != 0 && n % 64 <= 32;
   const int tm = fr1 ? 32 : 64, tn = fc1 ? 32 : 64;
   const dim3 grid(((m + tm - 1) / tm) * ((n + tn - 1) / tn), B);
   const bool sa = A.s != nullptr, sb = Bo.s != nullptr;
