@@ -10,6 +10,8 @@ import torch
 
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libdpnegf_amd.so")
 
+ABI_VERSION = 2  # DPNEGF_ABI_VERSION of include/dpnegf_amd.h this binding matches
+
 DPNEGF_FP64 = 0
 DPNEGF_MIXED = 1
 
@@ -18,6 +20,7 @@ STATUS_LS_NOCONV = 2
 STATUS_LS_RESID_FAIL = 4
 STATUS_LS_RESID_WARN = 8
 STATUS_REFINE_FALLBACK = 16
+STATUS_BAD_INDEX = 32
 
 TRANSMISSION_NMAX = 64
 LEAD_NMAX = 48
@@ -40,7 +43,7 @@ _SIGS = {
                                  _vp, _vp, _vp, _i32, _vp, _vp, _vp], _i32),
     "dpnegf_rgf_full_workspace_elems": ([_i32, _vp, _i32], _i64),
     "dpnegf_lead_self_energy_workspace_elems": ([_i32, _i32, _i32], _i64),
-    "dpnegf_rgf_full": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _i32] + [_vp] * 17,
+    "dpnegf_rgf_full": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _i32] + [_vp] * 18,
                         _i32),
     "dpnegf_rgf_grad_workspace_elems": ([_i32, _vp, _i32], _i64),
     "dpnegf_rgf_transmission_grad": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _i32,
@@ -82,6 +85,9 @@ def load():
             f = getattr(lib, name)
             f.argtypes = args
             f.restype = res
+        if lib.dpnegf_abi_version() != ABI_VERSION:
+            raise NativeError(f"{_LIB_PATH} has ABI version {lib.dpnegf_abi_version()}, this "
+                              f"binding expects {ABI_VERSION}: rebuild the library")
         _lib = lib
     return _lib
 

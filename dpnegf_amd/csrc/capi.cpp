@@ -93,7 +93,7 @@ int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h
     return fail(DPNEGF_ERR_ARG, "unknown precision %d", precision);
   negf::RgfArgs a;
   a.K = K; a.bsz = bsz; a.arena = (const cx<double>*)arena; a.sys_stride = sys_stride;
-  a.sys_idx = sys_idx; a.P = P; a.z = (const cx<double>*)z;
+  a.sys_idx = sys_idx; a.nsys = nsys; a.P = P; a.z = (const cx<double>*)z;
   a.sigL = (const cx<double>*)sigL; a.nL = nL; a.sigR = (const cx<double>*)sigR; a.nR = nR;
   a.T = T; a.status = status;
   bool uniform = true;
@@ -128,7 +128,8 @@ int dpnegf_lead_self_energy(void* stream, int n, int nred, int mode, const void*
   if (nsys > 1 && lead_stride < need) return fail(DPNEGF_ERR_ARG, "lead_stride too small");
   negf::LeadArgs a;
   a.n = n; a.nred = nred; a.mode = mode; a.lead = (const cx<double>*)lead;
-  a.lead_stride = lead_stride; a.sys_idx = sys_idx; a.P = P; a.e = (const cx<double>*)e;
+  a.lead_stride = lead_stride; a.sys_idx = sys_idx; a.nsys = nsys; a.P = P;
+  a.e = (const cx<double>*)e;
   a.eta = eta; a.eps = (cx<double>*)eps_ws; a.gs = (cx<double>*)gs; a.sigma = (cx<double>*)sigma;
   a.iters = iters; a.status = status; a.resid = resid; a.stop = stop;
   hipError_t err;
@@ -158,7 +159,7 @@ int64_t dpnegf_rgf_full_workspace_elems(int K, const int32_t* h_bsz, int B) {
 
 int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena, int B,
                     const void* z, const void* sigL, int nL, const void* sigR, int nR, int ortho,
-                    void* gl, void* grd, void* grl, void* gru, void* lc, void* g_trans,
+                    const void* seP, void* gl, void* grd, void* grl, void* gru, void* lc, void* g_trans,
                     const void* s_in, void* gin, void* gnd, void* gnl, void* gnu,
                     const void* s_out, void* gip, void* gpd, void* gpl, void* gpu, void* work,
                     int32_t* status) {
@@ -181,6 +182,7 @@ int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena
   a.K = K; a.B = B; a.nmax = nmax; a.ortho = ortho; a.bsz = h_bsz;
   a.arena = (const cx<double>*)arena; a.z = (const cx<double>*)z;
   a.sigL = (const cx<double>*)sigL; a.nL = nL; a.sigR = (const cx<double>*)sigR; a.nR = nR;
+  a.seP = (const cx<double>*)seP;
   a.gl = (cx<double>*)gl; a.grd = (cx<double>*)grd; a.grl = (cx<double>*)grl;
   a.gru = (cx<double>*)gru; a.lc = (cx<double>*)lc; a.g_trans = (cx<double>*)g_trans;
   a.s_in = (const cx<double>*)s_in; a.gin = (cx<double>*)gin; a.gnd = (cx<double>*)gnd;

@@ -95,6 +95,13 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
   if (pt >= a.P) return;
   const int n = a.n, nr = a.nred;
   const int sys = a.sys_idx ? a.sys_idx[pt] : 0;
+  if (sys < 0 || sys >= a.nsys) {  // uniform per workgroup, before any barrier
+    if (threadIdx.x == 0) {
+      if (a.iters) a.iters[pt] = 0;
+      if (a.status) a.status[pt] = STATUS_BAD_INDEX;
+    }
+    return;
+  }
   const cx<R>* H = a.lead + (int64_t)sys * a.lead_stride;
   const cx<R>* h01 = H + n * n;
   const cx<R>* S = h01 + n * n;

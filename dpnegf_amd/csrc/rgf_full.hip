@@ -39,8 +39,10 @@ __device__ __forceinline__ OpView op_view(const Opnd& o) {
 // Batched C = alpha op(A) op(B) + beta C: a TM x TN = 32 FR x 32 FC output tile
 // per workgroup (FR, FC in {1, 2}), wave w owning the FR x FC block of 16 x 16
 // tiles at rows 16 FR (w / 2), columns 16 FC (w % 2).  The host picks FR = 1
-// when the last 64-row tile would be at most half full (m mod 64 in (0, 32]),
-// and FC likewise, so the padding is at most 31 rows or columns.  K is staged
+// when m < 256 and the last 64-row tile would be at most half full (m mod 64
+// in (0, 32]), and FC likewise for n, so below 256 the padding is at most 31
+// rows or columns; from 256 on 64-wide tiles are kept, where a half-empty
+// last tile is at most 11% of the work.  K is staged
 // through LDS in chunks of KC = 16: the 256 threads load the A chunk (TM x 16,
 // thread t at column k = t % 16 of rows t / 16 + 16 u) and the B chunk
 // (16 x TN, row t / 16, columns t % 16 + 16 u), so sixteen consecutive lanes
@@ -236,10 +238,11 @@ __global__ __launch_bounds__(NT, MIN_WAVES) void binv_kernel(cx<double>* A, int 
   if (threadIdx.x == 0 && status && s) atomicOr(&status[blockIdx.x], STATUS_SINGULAR);
 }
 
-// D[b] = -(H - z_b S) - corner(SigmaL) - corner(SigmaR)
+// D[b] = -(H - z_b S) - Pb[b] - corner(SigmaL) - corner(SigmaR), Pb ([B, n, n],
+// the energy-dependent probe self-energy seP) optional
 __global__ void bbuild_kernel(cx<double>* D, const cx<double>* H, const cx<double>* S, int n,
                               const cx<double>* z, int B, const cx<double>* sL, int nL,
-                              const cx<double>* sR, int nR) {
+                              const cx<double>* sR, int nR, const cx<double>* Pb) {
   const int64_t tot = (int64_t)B * n * n;
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < tot;
        x += (int64_t)gridDim.x * blockDim.x) {
@@ -247,6 +250,7 @@ __global__ void bbuild_kernel(cx<double>* D, const cx<double>* H, const cx<doubl
     const int e = (int)(x % ((int64_t)n * n));
     const int i = e / n, k = e % n;
     cx<double> v = H[e] - z[b] * S[e];
+    if (Pb) v = v + Pb[x];
     if (sL && i < nL && k < nL) v = v + sL[(int64_t)b * nL * nL + i * nL + k];
     if (sR) {
       const int o = n - nR;
@@ -480,6 +484,19 @@ __global__ __launch_bounds__(256) void ls_conv_kernel(const cx<double>* al, cons
       atomicAdd(remaining, 1);
     }
   }
+}
+
+// out[b] = sys[b] when it names one of the nsys lead records, else 0 with
+// STATUS_BAD_INDEX or-ed into status[b]: every later kernel indexes records
+// through out, so no read leaves the lead arena.
+__global__ void ls_sys_check_kernel(int32_t* out, const int32_t* sys, int nsys, int32_t* status,
+                                    int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int s = sys[b];
+  const bool ok = s >= 0 && s < nsys;
+  out[b] = ok ? s : 0;
+  if (!ok && status) status[b] |= STATUS_BAD_INDEX;
 }
 
 __global__ void fill_i32_kernel(int32_t* p, int32_t v, int64_t count) {
@@ -758,7 +775,8 @@ hipError_t rgf_full(const RgfFullArgs& a, hipStream_t st) {
     const int64_t tot = (int64_t)B * n[q] * n[q];
     const int grid = (int)std::min<int64_t>((tot + bthreads - 1) / bthreads, 65535);
     hipLaunchKernelGGL(bbuild_kernel, dim3(grid), dim3(bthreads), 0, st, gl(q), hd(q), sdp(q), n[q],
-                       z, B, left ? a.sigL : nullptr, a.nL, right ? a.sigR : nullptr, a.nR);
+                       z, B, left ? a.sigL : nullptr, a.nL, right ? a.sigR : nullptr, a.nR,
+                       a.seP ? a.seP + od[q] : nullptr);
     return hipGetLastError();
   };
 
@@ -934,7 +952,7 @@ hipError_t rgf_transmission_ops(const RgfOpsArgs& a, hipStream_t st) {
   auto build = [&](int q, bool left, bool right) {
     hipLaunchKernelGGL(bbuild_kernel, dim3(grid_of((int64_t)B * n[q] * n[q])), dim3(256), 0, st, g,
                        hd(q), sdp(q), n[q], a.z, B, left ? a.sigL : nullptr, a.nL,
-                       right ? a.sigR : nullptr, a.nR);
+                       right ? a.sigR : nullptr, a.nR, nullptr);
     return hipGetLastError();
   };
   const int n0 = n[0];
@@ -990,11 +1008,13 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
   cx<double>* X1 = Tm + m2;
   cx<double>* X2 = X1 + m2;
   cx<double>* iws = X2 + m2;  // inverse workspace
-  // flags: done [B], iteration counts [B], remaining [1]
+  // flags: done [B], iteration counts [B], remaining [1], checked sys_idx [B]
+  // (3 B + 1 int32 of the 16 B the workspace reserves)
   int32_t* flags = (int32_t*)(iws + inverse_workspace(m, B));
   int32_t* done = flags;
   int32_t* iters = a.iters ? a.iters : flags + B;
   int32_t* remaining = flags + 2 * B;
+  int32_t* sysc = flags + 2 * B + 1;
   cx<double>* epss = a.gs;  // eps_s accumulates in the g_s output
   const cx<double> one{1, 0}, zero{0, 0}, mone{-1, 0};
   const int64_t nn = (int64_t)n * n;
@@ -1005,8 +1025,15 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
                      2 * (int64_t)B + 1);
   if (status) hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_of(B)), dim3(256), 0, st, status, 0, (int64_t)B);
   CHK(hipGetLastError());
+  const int32_t* sys_idx = nullptr;
+  if (a.sys_idx) {
+    hipLaunchKernelGGL(ls_sys_check_kernel, dim3((B + 255) / 256), dim3(256), 0, st, sysc, a.sys_idx,
+                       a.nsys, status, B);
+    CHK(hipGetLastError());
+    sys_idx = sysc;
+  }
   hipLaunchKernelGGL(ls_init_kernel, dim3(gridn), dim3(256), 0, st, al, be, eps, epss, a.lead,
-                     a.lead_stride, a.sys_idx, a.e, n, B, a.eta);
+                     a.lead_stride, sys_idx, a.e, n, B, a.eta);
   CHK(hipGetLastError());
   auto M = [&](cx<double>* p) { return mat(p, n, nn); };
   int it = 0;
@@ -1014,7 +1041,7 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
   while (true) {
     ++it;
     hipLaunchKernelGGL(ls_zs_minus_kernel, dim3(gridn), dim3(256), 0, st, W, eps, a.lead,
-                       a.lead_stride, a.sys_idx, a.e, n, B, a.eta);
+                       a.lead_stride, sys_idx, a.e, n, B, a.eta);
     CHK(hipGetLastError());
     // X_a = W^-1 alpha, X_b = W^-1 beta solved as the reference solves them (surface_green.py:189-194):
     // through the inverse, refined once in FP64, X += W^-1 (rhs - W X).  Without the
@@ -1057,21 +1084,21 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
   }
   // g_s = (z S - eps_s)^-1
   hipLaunchKernelGGL(ls_zs_minus_kernel, dim3(gridn), dim3(256), 0, st, W, epss, a.lead,
-                     a.lead_stride, a.sys_idx, a.e, n, B, a.eta);
+                     a.lead_stride, sys_idx, a.e, n, B, a.eta);
   CHK(hipGetLastError());
   CHK(inverse(st, W, n, B, status, iws));
   CHK(hipMemcpyAsync(a.gs, W, sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
   const cx<double>* gs = a.gs;
   // residual: X1 = (z s01 - h01) g_s ; X2 = X1 (z s10 - h10) ; R = (zS - H - X2) g_s
   hipLaunchKernelGGL(ls_coupling_kernel, dim3(gridn), dim3(256), 0, st, Ai, a.lead, a.lead_stride,
-                     a.sys_idx, a.e, a.eta, nn, 3 * nn, n, n, 0, B);
+                     sys_idx, a.e, a.eta, nn, 3 * nn, n, n, 0, B);
   hipLaunchKernelGGL(ls_coupling_kernel, dim3(gridn), dim3(256), 0, st, Bi, a.lead, a.lead_stride,
-                     a.sys_idx, a.e, a.eta, nn, 3 * nn, n, n, 1, B);
+                     sys_idx, a.e, a.eta, nn, 3 * nn, n, n, 1, B);
   CHK(hipGetLastError());
   CHK(gemm(st, B, n, n, n, M(Ai), mat(gs, n, nn), one, zero, X1, nn, n, nullptr));
   CHK(gemm(st, B, n, n, n, M(X1), M(Bi), one, zero, X2, nn, n, nullptr));
   hipLaunchKernelGGL(ls_test_kernel, dim3(gridn), dim3(256), 0, st, Tm, X2, a.lead, a.lead_stride,
-                     a.sys_idx, a.e, n, B, a.eta);
+                     sys_idx, a.e, n, B, a.eta);
   CHK(hipGetLastError());
   CHK(gemm(st, B, n, n, n, M(Tm), mat(gs, n, nn), one, zero, X1, nn, n, nullptr));
   hipLaunchKernelGGL(ls_resid_kernel, dim3(B), dim3(256), 0, st, X1, n, a.resid, status);
@@ -1081,9 +1108,9 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
   if (a.mode == 0) {
     // tau1 = e S_DL - H_DL [nr x n], tau2 = e S_DL^H - H_DL^H [n x nr]  (real-axis e: eta 0)
     hipLaunchKernelGGL(ls_coupling_kernel, dim3(grid_of((int64_t)B * nr * n)), dim3(256), 0, st, Ai,
-                       a.lead, a.lead_stride, a.sys_idx, a.e, 0.0, dl, sdl, nr, n, 0, B);
+                       a.lead, a.lead_stride, sys_idx, a.e, 0.0, dl, sdl, nr, n, 0, B);
     hipLaunchKernelGGL(ls_coupling_kernel, dim3(grid_of((int64_t)B * nr * n)), dim3(256), 0, st, Bi,
-                       a.lead, a.lead_stride, a.sys_idx, a.e, 0.0, dl, sdl, nr, n, 1, B);
+                       a.lead, a.lead_stride, sys_idx, a.e, 0.0, dl, sdl, nr, n, 1, B);
     CHK(hipGetLastError());
     CHK(gemm(st, B, n, nr, n, mat(gs, n, nn), mat(Bi, nr, (int64_t)n * nr), one, zero, X1,
              (int64_t)n * nr, nr, nullptr));
@@ -1095,7 +1122,7 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
     if (a.mode == 1) {
       // Sigma = (e S - H) - g_s^-1 : the coupling kernel with h = H (offset 0), s = S (2nn)
       hipLaunchKernelGGL(ls_coupling_kernel, dim3(gridn), dim3(256), 0, st, X1, a.lead,
-                         a.lead_stride, a.sys_idx, a.e, 0.0, (int64_t)0, 2 * nn, n, n, 0, B);
+                         a.lead_stride, sys_idx, a.e, 0.0, (int64_t)0, 2 * nn, n, n, 0, B);
       hipLaunchKernelGGL(bsub_kernel, dim3(gridn), dim3(256), 0, st, a.sigma, X1, W, (int64_t)B * nn);
       CHK(hipGetLastError());
     } else {
@@ -1328,7 +1355,7 @@ hipError_t rgf_grad_ckpt(const RgfGradArgs& g, int stride, hipStream_t st) {
     const int64_t tot = (int64_t)B * n[q] * n[q];
     const int grid = (int)std::min<int64_t>((tot + 255) / 256, 65535);
     hipLaunchKernelGGL(bbuild_kernel, dim3(grid), dim3(256), 0, st, dst, hd(q), sdp(q), n[q], g.z, B,
-                       q == 0 ? g.sigL : nullptr, g.nL, q == K - 1 ? g.sigR : nullptr, g.nR);
+                       q == 0 ? g.sigL : nullptr, g.nL, q == K - 1 ? g.sigR : nullptr, g.nR, nullptr);
     return hipGetLastError();
   };
   // g_{q+1} from g_q  (:198-207)

@@ -43,6 +43,7 @@ struct RgfFullArgs {
   const cx<double>* z;         // device [B]
   const cx<double>* sigL; int nL;
   const cx<double>* sigR; int nR;
+  const cx<double>* seP;       // [sum_q B n_q^2] energy-dependent diagonal addend, or null
   cx<double>* gl;              // [sum_q B n_q^2]          left-connected g_q
   cx<double>* grd;             // [sum_q B n_q^2]
   cx<double>* grl;             // [sum_q B n_q+1 n_q]

@@ -10,6 +10,7 @@ enum : int32_t {
   STATUS_LS_RESID_FAIL = 4,  // Lopez-Sancho: Dyson residual >= 3e-5
   STATUS_LS_RESID_WARN = 8,  // Lopez-Sancho: Dyson residual > 1e-8
   STATUS_REFINE_FALLBACK = 16,  // mixed precision: refinement did not converge
+  STATUS_BAD_INDEX = 32,     // sys_idx[p] outside [0, nsys): point skipped
 };
 
 // Largest per-thread row count of wg_gj_inverse over all n <= nmax.
@@ -36,6 +37,7 @@ struct RgfArgs {
   const cx<double>* arena;   // device, nsys systems
   int64_t sys_stride;        // elements between systems
   const int32_t* sys_idx;    // device [P] or null (all system 0)
+  int nsys;                  // systems in the arena (sys_idx range)
   int P;
   const cx<double>* z;       // device [P]: E + E_ref + i eta
   const cx<double>* sigL;    // device [P, nL, nL]
@@ -59,6 +61,7 @@ struct LeadArgs {
   const cx<double>* lead;    // per system: H, h01, S, s01 [n x n], hDL, sDL [nred x n]
   int64_t lead_stride;
   const int32_t* sys_idx;    // [P] or null
+  int nsys;                  // lead records (sys_idx range)
   int P;
   const cx<double>* e;       // [P] E + E_ref (no eta)
   double eta;                // eta_lead

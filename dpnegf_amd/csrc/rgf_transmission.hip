@@ -40,6 +40,13 @@ void rgf_transmission_kernel(RgfArgs a) {
   const int pt = blockIdx.x;
   if (pt >= a.P) return;  // uniform per workgroup
   const int sys = a.sys_idx ? a.sys_idx[pt] : 0;
+  if (sys < 0 || sys >= a.nsys) {  // uniform per workgroup, before any barrier
+    if (threadIdx.x == 0) {
+      a.T[pt] = __builtin_nan("");
+      if (a.status) a.status[pt] = STATUS_BAD_INDEX;
+    }
+    return;
+  }
   const cx<double>* base = a.arena + (int64_t)sys * a.sys_stride;
   const cx<double> zd = a.z[pt];
   const int K = a.K;

@@ -46,13 +46,31 @@ def pack_arena(hd, hu, hl, sd, su, sl, device):
     return torch.cat(parts)
 
 
-def pack_batched_blocks(blocks, B, device):
-    """List of [B, m, n] (or 2-D, broadcast) tensors -> one contiguous arena."""
+def batch_of(t, B, what="tensor"):
+    """``t`` as [B, m, n]: a 2-D matrix or a [1, m, n] batch is broadcast over
+    the B energies (as torch broadcasting does in the reference,
+    ``recursive_green_cal.py:397-428``); any other batch size is an error, so
+    no caller can hand the library fewer than B matrices."""
+    if t.ndim == 2:
+        return t.unsqueeze(0).expand(B, -1, -1)
+    if t.ndim != 3:
+        raise ValueError(f"{what} must be 2-D or [B, m, n] (got shape {tuple(t.shape)})")
+    if t.shape[0] == B:
+        return t
+    if t.shape[0] == 1:
+        return t.expand(B, -1, -1)
+    raise ValueError(f"{what} has batch size {t.shape[0]}, expected {B} energies")
+
+
+def pack_batched_blocks(blocks, B, device, shapes=None, what="block"):
+    """List of [B, m, n] (or 2-D / [1, m, n], broadcast) tensors -> one
+    contiguous arena; ``shapes`` (optional) are the required (m, n) per block."""
     out = []
-    for t in blocks:
-        t = t.to(device=device, dtype=C128)
-        if t.ndim == 2:
-            t = t.unsqueeze(0).expand(B, -1, -1)
+    for q, t in enumerate(blocks):
+        t = batch_of(t.to(device=device, dtype=C128), B, f"{what} {q}")
+        if shapes is not None and tuple(t.shape[-2:]) != tuple(shapes[q]):
+            raise ValueError(f"{what} {q} has shape {tuple(t.shape[-2:])}, expected "
+                             f"{tuple(shapes[q])}")
         out.append(t.contiguous().reshape(-1))
     return torch.cat(out)
 
@@ -70,8 +88,7 @@ def split_blocks(arena, B, shapes):
 def corner_self_energy(se, n_block, B, at_end, device):
     """Trim a self-energy to its corner block the way recursive_gf does
     (dpnegf/negf/recursive_green_cal.py:416-428); returns ([B, r, r], r)."""
-    if se.ndim == 2:
-        se = se.unsqueeze(0).expand(B, -1, -1)
+    se = batch_of(se, B, "self-energy")
     r = min(n_block, se.shape[-2])
     c = min(n_block, se.shape[-1])
     if r != c:

@@ -38,9 +38,19 @@ def gauss_xw(xl, xu, n=100):
     return x * (0.5 * (xu - xl)) + 0.5 * (xu + xl), w
 
 
+def _lead_sigma(lead, e, eta, E_ref):
+    """Self-energies [B, a, a] at the energies ``e`` (relative to ``E_ref``):
+    ``lead`` is either a tuple (hL, hLL, hDL, sL, sLL, sDL) or a callable
+    ``lead(e, eta)`` (a lead object's own route, which applies Bloch
+    unfolding when the lead uses it)."""
+    if callable(lead):
+        return lead(e, eta)
+    return self_energy_batched([lead], e, eta, E_ref=E_ref).sigma
+
+
 def _sigmas(lead_L, lead_R, e, eta, E_ref, nblk0, nblkK):
-    sL = self_energy_batched([lead_L], e, eta, E_ref=E_ref).sigma
-    sR = self_energy_batched([lead_R], e, eta, E_ref=E_ref).sigma
+    sL = _lead_sigma(lead_L, e, eta, E_ref)
+    sR = _lead_sigma(lead_R, e, eta, E_ref)
     B = e.shape[0]
     sL, nL = corner_self_energy(sL, nblk0, B, False, e.device)
     sR, nR = corner_self_energy(sR, nblkK, B, True, e.device)
@@ -69,7 +79,8 @@ def ozaki_density(blocks, lead_L, lead_R, kBT, mu_L, mu, R, poles, residues, E_r
     """Density matrices of every diagonal block of one k-point.
 
     ``blocks`` = (hd, hu, hl, sd, su, sl) CUDA tensors; ``lead_X`` = lead tuples
-    (hL, hLL, hDL, sL, sLL, sDL).  Energies are relative to ``E_ref``.
+    (hL, hLL, hDL, sL, sLL, sDL), or callables ``lead_X(e, eta)`` returning the
+    self-energies [B, a, a] at energies ``e``.  Energies are relative to ``E_ref``.
     Returns (DM_eq list, DM_neq list or 0.)."""
     hd, hu, hl, sd, su, sl = blocks
     dev = hd[0].device
@@ -163,8 +174,11 @@ class Ozaki(object):
         hs = deviceprop.hamiltonian.get_hs_device(kpoint, V, block_tridiagonal)
         dev = deviceprop.rgf_device
         hd, sd, hl, su, sl, hu = [[b.to(dev) for b in blk] for blk in hs]
+        # the leads' own batched route (Bloch-unfolded when they use it,
+        # lead_property.py:330-379), as the reference reaches them through
+        # lead.self_energy (density.py:167-177)
         dm_eq, dm_neq = ozaki_density(
-            (hd, hu, hl, sd, su, sl), lL._lead_blocks(kpoint), lR._lead_blocks(kpoint), kBT, mu_L,
+            (hd, hu, hl, sd, su, sl), lL.sigma_fn(kpoint), lR.sigma_fn(kpoint), kBT, mu_L,
             mu, self.R, self.poles, self.residues, deviceprop.E_ref, eta_lead, eta_device,
             lL.voltage, lR.voltage, self.n_gauss, lL.fermi_dirac, lR.fermi_dirac)
         if all_blocks:

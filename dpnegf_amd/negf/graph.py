@@ -116,6 +116,9 @@ class TransmissionGraph:
         for L, side in zip(self.leads, (leads_L, leads_R)):
             if side is None:
                 continue
+            if L["ortho"] and not all(_orthogonal(l[3].to(self.bs.device), l[4].to(self.bs.device))
+                                      for l in side):
+                raise ValueError("lead overlap pattern changed (captured as orthogonal)")
             recs = [pack_lead(l[0], l[1], l[3], l[4], l[2], l[5], self.bs.device)[0] for l in side]
             new = torch.cat(recs)
             if new.numel() != L["arena"].numel():

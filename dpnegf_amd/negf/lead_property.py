@@ -83,6 +83,19 @@ class LeadProperty(object):
             del self.HLk, self.HLLk, self.HDLk, self.SLk, self.SLLk, self.SDLk
         return self.se
 
+    def sigma_fn(self, kpoint):
+        """``f(e, eta)`` -> self-energies [B, a, a] at the energies ``e`` ([B],
+        complex allowed, relative to ``E_ref``) with decimation broadening
+        ``eta``, through the same route as ``self_energy_cal`` (Bloch-unfolded
+        when ``useBloch``).  Leaves ``self.se`` unchanged."""
+        def f(e, eta):
+            e = torch.as_tensor(e).reshape(-1)
+            if self.useBloch:
+                return self._bloch_self_energy(kpoint, e, eta)
+            lead = self._lead_blocks(kpoint)
+            return self_energy_batched([lead], e, eta, E_ref=self.E_ref).sigma
+        return f
+
     def _subblocks(self, kpoint):
         if hasattr(self.hamiltonian, "get_hs_device"):
             try:

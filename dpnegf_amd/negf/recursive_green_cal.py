@@ -66,12 +66,19 @@ def recursive_gf(energy, hl, hd, hu, sd, su, sl, left_se, right_se, seP=None, E_
             "need_greater=True requires s_out to be a list of coupling matrices"
 
     hd_eff = list(hd)
+    seP_blocks = None  # energy-dependent probe self-energies ([B, n, n] per block)
     if seP is not None:
+        if len(seP) != len(hd):
+            raise ValueError(f"seP has {len(seP)} blocks, the device {len(hd)}")
         for i in range(len(hd_eff)):
             sp = seP[i]
-            if torch.is_tensor(sp):
-                if sp.ndim != 2:
-                    raise NotImplementedError("energy-dependent seP blocks are not supported")
+            if torch.is_tensor(sp) and sp.ndim == 3 and sp.shape[0] == 1:
+                sp = sp[0]  # one matrix broadcast over the batch
+            if torch.is_tensor(sp) and sp.ndim == 3:
+                if seP_blocks is None:
+                    seP_blocks = [torch.zeros_like(h) for h in hd]
+                seP_blocks[i] = sp
+            elif torch.is_tensor(sp):
                 hd_eff[i] = hd_eff[i] + sp.to(device)
             elif sp != 0:
                 hd_eff[i] = hd_eff[i] + sp
@@ -86,9 +93,12 @@ def recursive_gf(energy, hl, hd, hu, sd, su, sl, left_se, right_se, seP=None, E_
     if isinstance(right_se, torch.Tensor):
         sigR, nR = corner_self_energy(right_se, sizes[-1], B, True, device)
 
+    sq = [(n, n) for n in sizes]
     raw = _run_full(z, arena, sizes, ortho, sigL, nL, sigR, nR, need_gr_lc,
-                    pack_batched_blocks(s_in, B, device) if need_lesser else None,
-                    pack_batched_blocks(s_out, B, device) if need_greater else None)
+                    pack_batched_blocks(s_in, B, device, sq, "s_in") if need_lesser else None,
+                    pack_batched_blocks(s_out, B, device, sq, "s_out") if need_greater else None,
+                    seP=(pack_batched_blocks(seP_blocks, B, device, sq, "seP")
+                         if seP_blocks is not None else None))
     sq_shapes = [(n, n) for n in sizes]
     lo_shapes = [(sizes[q + 1], sizes[q]) for q in range(K - 1)]
     up_shapes = [(sizes[q], sizes[q + 1]) for q in range(K - 1)]
@@ -137,8 +147,10 @@ def full_points_per_call(sizes, need_lc, P, device, mem_fraction=0.5):
     return max(1, min(P, fit, _MAX_B))
 
 
-def _run_full(z, arena, sizes, ortho, sigL, nL, sigR, nR, need_lc, s_in=None, s_out=None):
-    """One ``dpnegf_rgf_full`` call; returns the raw block-major arenas."""
+def _run_full(z, arena, sizes, ortho, sigL, nL, sigR, nR, need_lc, s_in=None, s_out=None,
+              seP=None):
+    """One ``dpnegf_rgf_full`` call; returns the raw block-major arenas.
+    ``seP``: optional per-energy diagonal addend, a block-major [B, n_q, n_q] arena."""
     device = z.device
     B = z.shape[0]
     K = len(sizes)
@@ -166,7 +178,7 @@ def _run_full(z, arena, sizes, ortho, sigL, nL, sigR, nR, need_lc, s_in=None, s_
         raise ValueError(nat.load().dpnegf_last_error().decode())
     work = torch.empty(nwork, dtype=C128, device=device)
     args = [nat.stream_ptr(device), K, nat.host_i32(sizes), nat.ptr(arena), B, nat.ptr(z),
-            nat.ptr(sigL), nL, nat.ptr(sigR), nR, int(ortho),
+            nat.ptr(sigL), nL, nat.ptr(sigR), nR, int(ortho), nat.ptr(seP),
             nat.ptr(out["gl"]), nat.ptr(out["grd"]), nat.ptr(out["grl"]), nat.ptr(out["gru"]),
             nat.ptr(out["lc"]), nat.ptr(out["g_trans"])]
     for part in (out["lesser"], out["greater"]):
@@ -201,6 +213,38 @@ class BTDSystems:
         self._h_bsz = nat.host_i32(self.sizes)
 
 
+def _check_points(systems, z, sigL, sigR, sys_idx):
+    """The per-point inputs of :func:`rgf_transmission` as the library reads
+    them: ``z`` [P] complex128, ``sigL`` / ``sigR`` [P, r, r] complex128 with r
+    at most the corner block, contiguous, on the systems' device, and
+    ``sys_idx`` [P] int32 with every entry naming one of the systems."""
+    dev = systems.device
+    for name, t in (("z", z), ("sigL", sigL), ("sigR", sigR)):
+        if not torch.is_tensor(t) or t.dtype != C128:
+            raise TypeError(f"{name} must be a complex128 tensor")
+        if t.device.type != dev.type or (dev.index is not None and t.device.index is not None
+                                         and t.device.index != dev.index):
+            raise ValueError(f"{name} is on {t.device}, the systems on {dev}")
+    if z.ndim != 1:
+        raise ValueError(f"z must be 1-D [P] (got shape {tuple(z.shape)})")
+    P = z.shape[0]
+    for name, t, nb in (("sigL", sigL, systems.sizes[0]), ("sigR", sigR, systems.sizes[-1])):
+        if t.ndim != 3 or t.shape[0] != P or t.shape[1] != t.shape[2] or not 1 <= t.shape[1] <= nb:
+            raise ValueError(f"{name} must be [P={P}, r, r] with r <= {nb} "
+                             f"(got shape {tuple(t.shape)})")
+    if sys_idx is not None:
+        if sys_idx.ndim != 1 or sys_idx.shape[0] != P:
+            raise ValueError(f"sys_idx must be [P={P}] (got shape {tuple(sys_idx.shape)})")
+        if sys_idx.dtype.is_floating_point or sys_idx.dtype.is_complex:
+            raise TypeError("sys_idx must be an integer tensor")
+        sys_idx = sys_idx.to(device=dev, dtype=torch.int32).contiguous()
+        if P and (int(sys_idx.min()) < 0 or int(sys_idx.max()) >= systems.nsys):
+            raise ValueError(f"sys_idx must lie in [0, {systems.nsys})")
+    elif systems.nsys > 1:
+        raise ValueError("sys_idx is required with more than one system")
+    return z.contiguous(), sigL.contiguous(), sigR.contiguous(), sys_idx
+
+
 def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", status=None, out=None,
                      chunk=None):
     """T for every point: ``z`` [P] complex (E + E_ref + i eta), ``sigL`` [P, nL, nL],
@@ -213,6 +257,7 @@ def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", sta
     if not isinstance(systems, BTDSystems):
         raise TypeError("systems must be a BTDSystems")
     dev = systems.device
+    z, sigL, sigR, sys_idx = _check_points(systems, z, sigL, sigR, sys_idx)
     P = z.shape[0]
     T = out if out is not None else torch.empty(P, dtype=torch.float64, device=dev)
     if max(systems.sizes) <= nat.TRANSMISSION_NMAX:

@@ -133,6 +133,9 @@ def check_status(res, e):
     if int(st.max().item()) == 0:
         return
     st = st.cpu()
+    bad = (st & nat.STATUS_BAD_INDEX) != 0
+    if bad.any():
+        raise ValueError(f"sys_idx outside the lead systems at {int(bad.sum())} points")
     bad = (st & nat.STATUS_LS_NOCONV) != 0
     if bad.any():
         raise RuntimeError("Lopez-scheme not converged.")

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define DPNEGF_ABI_VERSION 1
+#define DPNEGF_ABI_VERSION 2
 
 #define DPNEGF_OK 0
 #define DPNEGF_ERR_ARG (-1)
@@ -43,6 +43,7 @@ extern "C" {
 #define DPNEGF_STATUS_LS_RESID_FAIL 4
 #define DPNEGF_STATUS_LS_RESID_WARN 8
 #define DPNEGF_STATUS_REFINE_FALLBACK 16
+#define DPNEGF_STATUS_BAD_INDEX 32 /* sys_idx[p] outside [0, nsys): point skipped */
 
 int dpnegf_abi_version(void);
 const char* dpnegf_last_error(void);
@@ -56,7 +57,8 @@ int64_t dpnegf_arena_elems(int K, const int32_t* h_bsz);
  *   z[p]       = E + E_ref + i*eta_device                 (complex)
  *   sigL[p]    = left self-energy  [nL x nL], nL <= n_0, added to hd_0's top-left corner
  *   sigR[p]    = right self-energy [nR x nR], nR <= n_{K-1}, bottom-right of hd_{K-1}
- *   sys_idx[p] = system (k-point) of point p; arena + sys * sys_stride (null: all 0)
+ *   sys_idx[p] = system (k-point) of point p; arena + sys * sys_stride (null: all 0);
+ *                an index outside [0, nsys) gives T[p] = NaN and DPNEGF_STATUS_BAD_INDEX
  *   ortho      = 1 if every sd = I and su = sl = 0 (S blocks are then not read)
  *   precision  = DPNEGF_FP64 or DPNEGF_MIXED
  *   T[p]       = Re Tr[Gamma_L G_{0,K-1} Gamma_R G_{0,K-1}^dagger]
@@ -82,7 +84,9 @@ int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h
  *   eps_ws = workspace of dpnegf_lead_self_energy_workspace_elems(n, nred, P) complex
  *            elements; gs out [P x n x n]; sigma out [P x nred x nred]
  *   iters/status/resid per point (may be null); status bits LS_* replace the
- *   reference's RuntimeError / ArithmeticError / warning.
+ *   reference's RuntimeError / ArithmeticError / warning.  sys_idx[p] outside
+ *   [0, nsys) sets DPNEGF_STATUS_BAD_INDEX and no lead record outside the
+ *   arena is read.
  * n, nred <= DPNEGF_LEAD_NMAX run one fused workgroup per point; larger leads
  * (up to DPNEGF_FULL_NMAX, P <= 65535 per call) iterate batched block operations
  * until every point has converged. */
@@ -127,10 +131,12 @@ int64_t dpnegf_rgf_full_workspace_elems(int K, const int32_t* h_bsz, int B);
  *   lc (gr_lc, may be null)     : B x n_q x n_{K-1}
  *   g_trans                     : B x n_0 x n_{K-1}
  *   s_in / s_out (may be null: no lesser / greater): B x n_q x n_q arenas
+ *   seP (may be null): energy-dependent addend of every diagonal block,
+ *                 B x n_q x n_q arena (the 3-D seP of recursive_gf, :411-414)
  * Block sizes up to DPNEGF_FULL_NMAX; B <= 65535 per call. */
 int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena, int B,
                     const void* z, const void* sigL, int nL, const void* sigR, int nR, int ortho,
-                    void* gl, void* grd, void* grl, void* gru, void* lc, void* g_trans,
+                    const void* seP, void* gl, void* grd, void* grl, void* gru, void* lc, void* g_trans,
                     const void* s_in, void* gin, void* gnd, void* gnl, void* gnu,
                     const void* s_out, void* gip, void* gpd, void* gpl, void* gpu, void* work,
                     int32_t* status);

@@ -20,15 +20,23 @@ Module map (reference file:line each restatement follows):
 
 Pinning status (what each function is checked against in ``tests/``):
 
-* ``rgf.recursive_gf`` retarded outputs: dense brute-force inverse, the
-  reference's batched==scalar self-consistency, and the reference's 4-site
+* ``rgf.recursive_gf``, all 14 outputs: the reference's own ``recursive_gf``
+  outputs on seeded cases (``tests/golden/reference_rgf.npz``, written by
+  ``tests/golden/make_reference_rgf.py``, which loads
+  ``dpnegf/negf/recursive_green_cal.py`` by file path) -- uneven blocks with
+  S_offdiag != 0 and complex energies, the stacked uniform path, a 3-D seP,
+  K = 1 and the scalar squeeze, greater branch (``.conj()`` quirk,
+  ``recursive_green_cal.py:145-160``) and lesser branch (``L_q`` quirk,
+  ``:134-139``) included; agreement is bit-exact.
+* ``rgf.recursive_gf`` retarded outputs: also the dense brute-force inverse,
+  the reference's batched==scalar self-consistency, and the reference's 4-site
   chain golden values (``dpnegf/tests/test_negf_device_property.py:103-186``).
-* ``rgf.recursive_gf`` lesser outputs: dense ``G Sigma^< G^dagger`` (exact for
-  Hermitian H, S_offdiag = 0) and the chain golden ``gnd``/``gin_left``.
-* ``rgf.recursive_gf`` greater outputs (quirk ``.conj()`` instead of ``.mH``,
-  ``recursive_green_cal.py:145-160``): **parity unpinned** -- no closed form
-  reproduces the quirk and the reference is never called with
-  ``need_greater=True``.
+* ``rgf.recursive_gf`` lesser / greater outputs: also the dense
+  ``G Sigma G^dagger`` where the quirks are exact (Hermitian H, S_offdiag = 0;
+  for greater, real symmetric H and complex-symmetric Sigma).
+* ``lead.HDL_reduced``: trimming keeps the coupling's self-energy (the trimmed
+  one is the corner of the untrimmed one, zero elsewhere), both branches, with
+  and without ``subblocks``.
 * ``surface_green``: closed-form 1-orbital chain, the Dyson fixed-point
   residual, the GEP route and a long finite chain.
 * ``ozaki``: Matsubara limit / Fermi-function identity, brute-force real-axis

@@ -27,7 +27,8 @@ def test_library_exports_every_declared_symbol():
     for name in _declared():
         assert hasattr(lib, name), name
     nat.load()
-    assert nat.load().dpnegf_abi_version() == 1
+    declared = int(re.search(r"#define DPNEGF_ABI_VERSION (\d+)", open(HEADER).read()).group(1))
+    assert nat.load().dpnegf_abi_version() == declared == nat.ABI_VERSION
 
 
 @pytest.mark.skipif(not os.path.exists(nat.lib_path()), reason="library not built")
@@ -43,7 +44,7 @@ def test_argument_errors_are_reported_without_a_device():
     assert rc == -1
     with pytest.raises(ValueError):
         nat.call("dpnegf_rgf_full", None, 0, None, None, 1, None, None, 0, None, 0, 1,
-                 *([None] * 17))
+                 *([None] * 18))
     for n in (0, 129):
         assert lib.dpnegf_zinv_batched(None, None, n, 1, None) < 0
         assert b"outside" in lib.dpnegf_last_error()

@@ -151,3 +151,67 @@ def test_ozaki_integrate_through_device_and_lead_objects():
     scale = max(float(np.abs(np.asarray(r)).max()) for r in ref_neq)
     for q in range(len(neq2)):
         assert (neq2[q].cpu() - torch.as_tensor(ref_neq[q])).abs().max().item() < 1e-10 * scale
+
+
+class _BlochHam:
+    """A device of uneven blocks between two Bloch-unfoldable leads: the
+    Hamiltonian object hands out the small cell at the unfolded points
+    (``supercell=False``) or the whole supercell lead (``supercell=True``)."""
+
+    def __init__(self, supercell):
+        self.sizes = [7, 9, 7]
+        self.blocks = wl.random_btd(self.sizes, seed=31)
+        self.leads = {"lead_L": wl.BlochLead(m=2, n=3, seed=32, a_dev=7),
+                      "lead_R": wl.BlochLead(m=2, n=3, seed=33, a_dev=7)}
+        self.supercell = supercell
+        self.device_norbs = [1] * sum(self.sizes)
+
+    def get_hs_device(self, kpoint, V=None, block_tridiagonal=True, only_subblocks=False):
+        if only_subblocks:
+            return None
+        hd, sd, hl, hu, sl, su = self.blocks
+        return [list(hd), list(sd), list(hl), list(su), list(sl), list(hu)]
+
+    def get_hs_lead(self, kpoint, tab, v):
+        lead = self.leads[tab]
+        if not self.supercell:
+            return lead.get_hs_lead(kpoint, v=v)
+        hL, hLL, sL, sLL = lead.supercell(np.asarray(kpoint, dtype=float))
+        t = lambda x: torch.as_tensor(np.asarray(x), dtype=torch.complex128)
+        return (t(hL - v * sL), t(hLL - v * sLL), t(lead.hdl), t(sL), t(sLL), t(lead.sdl))
+
+
+def test_ozaki_integrate_with_bloch_leads():
+    """``Ozaki.integrate`` with Bloch-unfolded leads (``useBloch``,
+    ``lead_property.py:330-379``) equals the same density with the leads
+    decimated whole as supercells (Bloch's theorem, which pins the unfolding
+    in test_oracle_leads_ozaki.py), equilibrium and bias window, every block,
+    at 1e-10; k = 0.3 along the unfolded direction."""
+    from dpnegf_amd.negf.density import Ozaki
+    from dpnegf_amd.negf.device_property import DeviceProperty
+    from dpnegf_amd.negf.lead_property import LeadProperty
+    dev = _dev()
+    kpt = [0.0, 0.3, 0.0]
+    out = {}
+    for supercell in (False, True):
+        ham = _BlochHam(supercell)
+        leads = []
+        for tab, v in (("lead_L", 0.04), ("lead_R", -0.04)):
+            lead = ham.leads[tab]
+            extra = {} if supercell else dict(
+                useBloch=True, bloch_factor=lead.bloch_factor, bloch_R_list=lead.R_list,
+                bloch_sorted_indice=lead.sorted_idx)
+            leads.append(LeadProperty(tab, ham, None, None, v, e_T=300, efermi=-0.1, device=dev,
+                                      **extra))
+        dp = DeviceProperty(ham, None, None, e_T=300, efermi=-0.1, rgf_device=dev)
+        dp.set_leadLR(*leads)
+        out[supercell] = Ozaki(1e6, 30, 8).integrate(dp, kpt, eta_lead=1e-5, eta_device=1e-6,
+                                                    all_blocks=True)
+    (eq_b, neq_b), (eq_s, neq_s) = out[False], out[True]
+    scale = max(x.abs().max().item() for x in eq_s)
+    for q in range(3):
+        assert (eq_b[q] - eq_s[q]).abs().max().item() < 1e-10 * scale, q
+    scale = max(x.abs().max().item() for x in neq_s)
+    assert scale > 0
+    for q in range(3):
+        assert (neq_b[q] - neq_s[q]).abs().max().item() < 1e-10 * scale, q
