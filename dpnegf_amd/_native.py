@@ -25,6 +25,7 @@ STATUS_BAD_INDEX = 32
 TRANSMISSION_NMAX = 64
 LEAD_NMAX = 48
 FULL_NMAX = 1024
+NOALLOC_SCRATCH_ELEMS = 8388608
 
 _lib = None
 
@@ -41,6 +42,8 @@ _SIGS = {
                                  _i32, _vp, _i32, _i32, _vp, _vp], _i32),
     "dpnegf_lead_self_energy": ([_vp, _i32, _i32, _i32, _vp, _i64, _i32, _vp, _i32, _vp, _f64, _f64,
                                  _vp, _vp, _vp, _i32, _vp, _vp, _vp], _i32),
+    "dpnegf_lead_self_energy_noalloc": ([_vp, _i32, _i32, _i32, _vp, _i64, _i32, _vp, _i32, _vp,
+                                         _f64, _f64, _vp, _vp, _i32, _vp, _vp, _vp], _i32),
     "dpnegf_rgf_full_workspace_elems": ([_i32, _vp, _i32], _i64),
     "dpnegf_lead_self_energy_workspace_elems": ([_i32, _i32, _i32], _i64),
     "dpnegf_rgf_full": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _i32] + [_vp] * 18,

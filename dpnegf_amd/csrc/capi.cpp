@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "dpnegf_amd.h"
@@ -107,11 +108,12 @@ int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h
   return 0;
 }
 
-int dpnegf_lead_self_energy(void* stream, int n, int nred, int mode, const void* lead,
-                            int64_t lead_stride, int nsys, const int32_t* sys_idx, int P,
-                            const void* e, double eta, double stop, void* eps_ws, void* gs,
-                            void* sigma, int ortho, int32_t* iters, int32_t* status,
-                            double* resid) {
+// Arguments of both lead entries; 1 when there is nothing to do (P == 0),
+// < 0 on an argument error, 0 otherwise.
+static int lead_args(negf::LeadArgs& a, int n, int nred, int mode, const void* lead,
+                     int64_t lead_stride, int nsys, const int32_t* sys_idx, int P, const void* e,
+                     double eta, double stop, void* gs, void* sigma, int32_t* iters,
+                     int32_t* status, double* resid) {
   if (n < 1 || nred < 1) return fail(DPNEGF_ERR_ARG, "bad lead sizes n=%d nred=%d", n, nred);
   if (n > DPNEGF_FULL_NMAX || nred > DPNEGF_FULL_NMAX)
     return fail(DPNEGF_ERR_UNSUPPORTED, "lead size %d / coupling rows %d exceed %d", n, nred,
@@ -119,35 +121,147 @@ int dpnegf_lead_self_energy(void* stream, int n, int nred, int mode, const void*
   if (mode < 0 || mode > 2) return fail(DPNEGF_ERR_ARG, "mode must be 0, 1 or 2");
   if (mode != 0 && nred != n) return fail(DPNEGF_ERR_ARG, "modes 1/2 need nred == n");
   if (P < 0) return fail(DPNEGF_ERR_ARG, "P must be >= 0");
-  if (P == 0) return 0;
-  if (!lead || !e || !eps_ws || !gs || !sigma) return fail(DPNEGF_ERR_ARG, "null device pointer");
+  if (P == 0) return 1;
+  if (!lead || !e || !gs || !sigma) return fail(DPNEGF_ERR_ARG, "null device pointer");
   if (nsys < 1) return fail(DPNEGF_ERR_ARG, "nsys must be >= 1");
   if (nsys > 1 && !sys_idx) return fail(DPNEGF_ERR_ARG, "sys_idx required when nsys > 1");
   // records: H, H01, S, S01 (n x n), then H_DL, S_DL (nred x n) in mode 0 only
   const int64_t need = 4 * (int64_t)n * n + (mode == 0 ? 2 * (int64_t)nred * n : 0);
   if (nsys > 1 && lead_stride < need) return fail(DPNEGF_ERR_ARG, "lead_stride too small");
-  negf::LeadArgs a;
   a.n = n; a.nred = nred; a.mode = mode; a.lead = (const cx<double>*)lead;
   a.lead_stride = lead_stride; a.sys_idx = sys_idx; a.nsys = nsys; a.P = P;
   a.e = (const cx<double>*)e;
-  a.eta = eta; a.eps = (cx<double>*)eps_ws; a.gs = (cx<double>*)gs; a.sigma = (cx<double>*)sigma;
+  a.eta = eta; a.eps = nullptr; a.gs = (cx<double>*)gs; a.sigma = (cx<double>*)sigma;
   a.iters = iters; a.status = status; a.resid = resid; a.stop = stop;
-  hipError_t err;
-  if (n <= DPNEGF_LEAD_NMAX && nred <= DPNEGF_LEAD_NMAX) {
-    err = negf::launch_lead_self_energy(a, ortho != 0, (hipStream_t)stream);
-  } else {
-    if (P > 65535) return fail(DPNEGF_ERR_ARG, "P=%d exceeds 65535 per call for blocks > %d", P,
-                               DPNEGF_LEAD_NMAX);
-    err = negf::lead_ls_batched(a, (cx<double>*)eps_ws, (hipStream_t)stream);
+  return 0;
+}
+
+static bool lead_fused(int n, int nred) {
+  return n <= DPNEGF_LEAD_NMAX && nred <= DPNEGF_LEAD_NMAX;
+}
+
+static int64_t lead_workspace(int n, int nred, int P) {
+  return lead_fused(n, nred) ? negf::lead_self_energy_workspace(n, P)
+                             : negf::lead_ls_batched_workspace(n, nred, P);
+}
+
+// One launch (fused kernel) or one batched decimation over the points of a.
+static hipError_t lead_run(const negf::LeadArgs& a, cx<double>* ws, int ortho, hipStream_t st) {
+  if (lead_fused(a.n, a.nred)) {
+    negf::LeadArgs b = a;
+    b.eps = ws;
+    return negf::launch_lead_self_energy(b, ortho != 0, st);
   }
+  return negf::lead_ls_batched(a, ws, st);
+}
+
+int dpnegf_lead_self_energy(void* stream, int n, int nred, int mode, const void* lead,
+                            int64_t lead_stride, int nsys, const int32_t* sys_idx, int P,
+                            const void* e, double eta, double stop, void* eps_ws, void* gs,
+                            void* sigma, int ortho, int32_t* iters, int32_t* status,
+                            double* resid) {
+  negf::LeadArgs a;
+  const int r = lead_args(a, n, nred, mode, lead, lead_stride, nsys, sys_idx, P, e, eta, stop, gs,
+                          sigma, iters, status, resid);
+  if (r) return r < 0 ? r : 0;
+  if (!eps_ws) return fail(DPNEGF_ERR_ARG, "null device pointer");
+  if (!lead_fused(n, nred) && P > 65535)
+    return fail(DPNEGF_ERR_ARG, "P=%d exceeds 65535 per call for blocks > %d", P,
+                DPNEGF_LEAD_NMAX);
+  hipError_t err = lead_run(a, (cx<double>*)eps_ws, ortho, (hipStream_t)stream);
   if (err != hipSuccess) return hip_fail(err, "lead_self_energy launch");
+  return 0;
+}
+
+// ---- no-allocation lead entry ---------------------------------------------------
+// The decimation workspace of dpnegf_lead_self_energy_noalloc lives in this
+// file-scope device array; every GPU the library runs on holds its own copy
+// (DPNEGF_NOALLOC_SCRATCH_ELEMS complex128, 128 MiB) until the process ends.
+// Host code never passes `scratch` itself to a kernel (on the host it is a
+// stand-in, not a device address): it uses the per-device address that
+// hipGetSymbolAddress returns, looked up once per device on first use.
+static __device__ __align__(256) cx<double> scratch[DPNEGF_NOALLOC_SCRATCH_ELEMS];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+cx<double>* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the stream's device (the current device for
+// the null stream), looked up with that device current.
+int scratch_address(hipStream_t st, cx<double>** out) {
+  int cur = 0;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  int dev = cur;
+  if (st) {
+    e = hipStreamGetDevice(st, &dev);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamGetDevice");
+  }
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+    return fail(DPNEGF_ERR_UNSUPPORTED, "device %d: the scratch table covers devices 0..%d", dev,
+                SCRATCH_MAX_DEVICES - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    if (dev != cur && (e = hipSetDevice(dev)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (dev != cur) {
+      const hipError_t e2 = hipSetDevice(cur);
+      if (e == hipSuccess && e2 != hipSuccess) return hip_fail(e2, "hipSetDevice");
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = (cx<double>*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+int dpnegf_lead_self_energy_noalloc(void* stream, int n, int nred, int mode, const void* lead,
+                                    int64_t lead_stride, int nsys, const int32_t* sys_idx, int P,
+                                    const void* e, double eta, double stop, void* gs, void* sigma,
+                                    int ortho, int32_t* iters, int32_t* status, double* resid) {
+  negf::LeadArgs a;
+  const int r = lead_args(a, n, nred, mode, lead, lead_stride, nsys, sys_idx, P, e, eta, stop, gs,
+                          sigma, iters, status, resid);
+  if (r) return r < 0 ? r : 0;
+  // points per launch: the workspace is linear in the point count
+  const int64_t per = lead_workspace(n, nred, 1);
+  int64_t cap = DPNEGF_NOALLOC_SCRATCH_ELEMS / per;
+  if (!lead_fused(n, nred) && cap > 65535) cap = 65535;
+  if (cap < 1)
+    return fail(DPNEGF_ERR_UNSUPPORTED,
+                "lead of %d orbitals (%d coupling rows) needs %lld workspace elements per point; "
+                "the no-allocation scratch holds %lld",
+                n, nred, (long long)per, (long long)DPNEGF_NOALLOC_SCRATCH_ELEMS);
+  cx<double>* ws = nullptr;
+  if (int rc = scratch_address((hipStream_t)stream, &ws)) return rc;
+  // equal chunks, rounded up to whole waves of 256 workgroups where that fits
+  const int64_t nchunk = (P + cap - 1) / cap;
+  int64_t chunk = (P + nchunk - 1) / nchunk;
+  if (nchunk > 1 && (chunk + 255) / 256 * 256 <= cap) chunk = (chunk + 255) / 256 * 256;
+  for (int64_t c0 = 0; c0 < P; c0 += chunk) {
+    const int64_t c1 = c0 + chunk < P ? c0 + chunk : P;
+    negf::LeadArgs b = a;
+    b.P = (int)(c1 - c0);
+    b.sys_idx = a.sys_idx ? a.sys_idx + c0 : nullptr;
+    b.e = a.e + c0;
+    b.gs = a.gs + c0 * n * n;
+    b.sigma = a.sigma + c0 * nred * nred;
+    b.iters = a.iters ? a.iters + c0 : nullptr;
+    b.status = a.status ? a.status + c0 : nullptr;
+    b.resid = a.resid ? a.resid + c0 : nullptr;
+    // stream order keeps each chunk's use of the scratch after the previous one's
+    hipError_t err = lead_run(b, ws, ortho, (hipStream_t)stream);
+    if (err != hipSuccess) return hip_fail(err, "lead_self_energy_noalloc launch");
+  }
   return 0;
 }
 
 int64_t dpnegf_lead_self_energy_workspace_elems(int n, int nred, int P) {
   if (n < 1 || nred < 1 || P < 0) return -1;
-  if (n <= DPNEGF_LEAD_NMAX && nred <= DPNEGF_LEAD_NMAX) return negf::lead_self_energy_workspace(n, P);
-  return negf::lead_ls_batched_workspace(n, nred, P);
+  return lead_workspace(n, nred, P);
 }
 
 int64_t dpnegf_rgf_full_workspace_elems(int K, const int32_t* h_bsz, int B) {

@@ -91,6 +91,28 @@ int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h
  * (up to DPNEGF_FULL_NMAX, P <= 65535 per call) iterate batched block operations
  * until every point has converged. */
 int64_t dpnegf_lead_self_energy_workspace_elems(int n, int nred, int P);
+
+/* Complex128 elements of the device scratch dpnegf_lead_self_energy_noalloc
+ * uses (128 MiB per GPU the library runs on, held until the process ends). */
+#define DPNEGF_NOALLOC_SCRATCH_ELEMS 8388608
+
+/* dpnegf_lead_self_energy without the eps_ws argument: the same kernels and
+ * results, with the decimation workspace in a file-scope device array of
+ * DPNEGF_NOALLOC_SCRATCH_ELEMS elements instead of a caller buffer, so the
+ * call allocates nothing (no hipMalloc, no caller workspace).  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * it).  Points are processed in stream-ordered chunks that fit the scratch;
+ * a lead whose single point needs more than the scratch (above about 830
+ * orbitals) is rejected with DPNEGF_ERR_UNSUPPORTED.  Every call on one GPU
+ * shares that GPU's scratch: calls that overlap on one GPU (concurrent
+ * streams or host threads) race and must be serialised by the caller.  The
+ * stream's device must be the one whose scratch is used; its address is
+ * looked up once per device on the first call there. */
+int dpnegf_lead_self_energy_noalloc(void* stream, int n, int nred, int mode, const void* lead,
+                                    int64_t lead_stride, int nsys, const int32_t* sys_idx, int P,
+                                    const void* e, double eta, double stop, void* gs, void* sigma,
+                                    int ortho, int32_t* iters, int32_t* status, double* resid);
 int dpnegf_lead_self_energy(void* stream, int n, int nred, int mode, const void* lead,
                             int64_t lead_stride, int nsys, const int32_t* sys_idx, int P,
                             const void* e, double eta, double stop, void* eps_ws, void* gs,
