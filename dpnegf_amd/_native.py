@@ -14,6 +14,7 @@ ABI_VERSION = 2  # DPNEGF_ABI_VERSION of include/dpnegf_amd.h this binding match
 
 DPNEGF_FP64 = 0
 DPNEGF_MIXED = 1
+DPNEGF_MIXED_REFINED = 2
 
 STATUS_SINGULAR = 1
 STATUS_LS_NOCONV = 2

@@ -90,8 +90,11 @@ int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h
   if (nsys > 1 && !sys_idx) return fail(DPNEGF_ERR_ARG, "sys_idx required when nsys > 1");
   if (nsys > 1 && sys_stride < arena_elems(K, h_bsz))
     return fail(DPNEGF_ERR_ARG, "sys_stride smaller than one system's arena");
-  if (precision != DPNEGF_FP64 && precision != DPNEGF_MIXED)
+  if (precision != DPNEGF_FP64 && precision != DPNEGF_MIXED && precision != DPNEGF_MIXED_REFINED)
     return fail(DPNEGF_ERR_ARG, "unknown precision %d", precision);
+  if (precision == DPNEGF_MIXED_REFINED && nmax > negf::REFINED_NMAX)
+    return fail(DPNEGF_ERR_UNSUPPORTED, "refined mixed precision needs blocks <= %d (got %d)",
+                negf::REFINED_NMAX, nmax);
   negf::RgfArgs a;
   a.K = K; a.bsz = bsz; a.arena = (const cx<double>*)arena; a.sys_stride = sys_stride;
   a.sys_idx = sys_idx; a.nsys = nsys; a.P = P; a.z = (const cx<double>*)z;
@@ -102,8 +105,10 @@ int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h
   hipError_t e;
   if (precision == DPNEGF_FP64)
     e = negf::launch_rgf_transmission<double>(a, nmax, uniform, ortho != 0, (hipStream_t)stream);
-  else
+  else if (precision == DPNEGF_MIXED)
     e = negf::launch_rgf_transmission_mixed(a, nmax, uniform, ortho != 0, (hipStream_t)stream);
+  else
+    e = negf::launch_rgf_transmission_refined(a, nmax, uniform, ortho != 0, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "rgf_transmission launch");
   return 0;
 }

@@ -53,6 +53,10 @@ hipError_t launch_rgf_transmission(const RgfArgs& a, int nmax, bool uniform, boo
                                    hipStream_t st);
 hipError_t launch_rgf_transmission_mixed(const RgfArgs& a, int nmax, bool uniform, bool ortho,
                                          hipStream_t st);
+// FP32 block inverses refined by one FP64 Newton step, FP64 products; nmax <= 48.
+hipError_t launch_rgf_transmission_refined(const RgfArgs& a, int nmax, bool uniform, bool ortho,
+                                           hipStream_t st);
+constexpr int REFINED_NMAX = 48;
 
 struct LeadArgs {
   int n;                     // principal-layer size

@@ -21,12 +21,23 @@
 
 namespace negf {
 
+// Residual max |I - A X0| above which the FP32 block inverse X0 is not
+// refined but replaced by the FP64 inverse (one Newton step squares the
+// relative error: 1e-4 -> about 1e-8 before the FP64 rounding floor).
+constexpr double REFINE_TOL = 1e-4;
+
 // FIXED: every block has exactly NMAX rows (compile-time shapes for the inverse
 // and the products).  Up to NMAX = 48 two workgroups share a CU (LDS), so the
 // register budget is held to two waves per SIMD.
-template <typename R, int NMAX, bool ORTHO, bool FIXED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NMAX <= 48 ? 2 : 1)))
+// REFINE (with R = double): every block inverse is computed in FP32 and
+// refined by one FP64 Newton step, X = X0 + X0 (I - A X0); a block whose FP32
+// residual exceeds REFINE_TOL is inverted in FP64 instead (status bit
+// STATUS_REFINE_FALLBACK).  Products and the carried P stay FP64.
+template <typename R, int NMAX, bool ORTHO, bool FIXED, bool REFINE = false>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(NMAX <= 48 && !REFINE ? 2 : 1)))
 void rgf_transmission_kernel(RgfArgs a) {
+  static_assert(!REFINE || (sizeof(R) == 8 && NMAX <= 48), "refinement: FP64 kernel, NMAX <= 48");
   constexpr int LD = NMAX + 1;
   constexpr int NT = (NMAX / 16) * (NMAX / 16);
   constexpr int MAXT_STACK = (2 * NT + 3) / 4;
@@ -36,6 +47,13 @@ void rgf_transmission_kernel(RgfArgs a) {
   __shared__ GJScratch<R, NMAX> gjs;
   __shared__ GJBScratch<R, NMAX> gjb;
   __shared__ double red[8];
+  // FP32 copy / inverse of the block and the FP64 residual (refined variant only)
+  constexpr int NR = REFINE ? NMAX * LD : 1;
+  __shared__ cx<float> XF[NR];
+  __shared__ cx<double> RS[NR];
+  __shared__ GJScratch<float, REFINE ? NMAX : 16> gjsf;
+  __shared__ GJBScratch<float, REFINE ? NMAX : 16> gjbf;
+  int refine_fallback = 0;
 
   const int pt = blockIdx.x;
   if (pt >= a.P) return;  // uniform per workgroup
@@ -79,7 +97,7 @@ void rgf_transmission_kernel(RgfArgs a) {
   const cx<double>* hd0 = base;
   const cx<double>* sd0 = base + n0 * n0;
   // FIXED: blocked inverse of g in place; otherwise the register-resident one
-  auto inverse_g = [&](int n) __attribute__((always_inline)) -> int {
+  auto inverse_full = [&](int n) __attribute__((always_inline)) -> int {
     if constexpr (FIXED)
       return wg_gj_inverse_blocked<R, 256, NMAX, LD>(
           g, [&](int i, int k, cx<R> v) { g[i * LD + k] = v; }, &gjb);
@@ -87,6 +105,51 @@ void rgf_transmission_kernel(RgfArgs a) {
       return wg_gj_inverse<R, 256, NMAX, MAXR>(
           n, [&](int i, int k) { return g[i * LD + k]; },
           [&](int i, int k, cx<R> v) { g[i * LD + k] = v; }, &gjs);
+  };
+  // g <- g^-1 in place; every thread calls it, and it ends with a barrier
+  auto inverse_g = [&](int n) __attribute__((always_inline)) -> int {
+    if constexpr (!REFINE) {
+      return inverse_full(n);
+    } else {
+      for (int e = threadIdx.x; e < n * n; e += 256) {
+        const int i = e / n, k = e % n;
+        XF[i * LD + k] = cvt<float>(g[i * LD + k]);
+      }
+      __syncthreads();
+      int sg;
+      if constexpr (FIXED)
+        sg = wg_gj_inverse_blocked<float, 256, NMAX, LD>(
+            XF, [&](int i, int k, cx<float> v) { XF[i * LD + k] = v; }, &gjbf);
+      else
+        sg = wg_gj_inverse<float, 256, NMAX, gj_maxr(NMAX, 256)>(
+            n, [&](int i, int k) { return XF[i * LD + k]; },
+            [&](int i, int k, cx<float> v) { XF[i * LD + k] = v; }, &gjsf);
+      __syncthreads();
+      // RS = I - A X0 in FP64 (A still in g), and its largest entry
+      double rmax = 0.0;
+      auto ra = [&](int i, int k) { return g[i * LD + k]; };
+      auto rb = [&](int k, int j) { return cvt<double>(XF[k * LD + j]); };
+      auto re = [&](int i, int j, cx<double> v) {
+        const cx<double> r = cx<double>{(i == j ? 1.0 : 0.0) - v.re, -v.im};
+        RS[i * LD + j] = r;
+        rmax = fmax(rmax, abs1(r));
+      };
+      if constexpr (FIXED) wg_gemm_fixed<double, 4, NMAX, NMAX, NMAX, false, false>(ra, rb, re);
+      else wg_gemm<double, 4, MAXT_SQ>(n, n, n, ra, rb, re);
+      rmax = block_max(rmax, red);  // ends with the residual visible to all
+      if (sg || !(rmax < REFINE_TOL)) {  // uniform: singular, NaN or too far for one step
+        refine_fallback = 1;
+        return inverse_full(n);
+      }
+      // g <- X0 + X0 RS
+      auto xa = [&](int i, int k) { return cvt<double>(XF[i * LD + k]); };
+      auto xb = [&](int k, int j) { return RS[k * LD + j]; };
+      auto xe = [&](int i, int j, cx<double> v) { g[i * LD + j] = cvt<double>(XF[i * LD + j]) + v; };
+      if constexpr (FIXED) wg_gemm_fixed<double, 4, NMAX, NMAX, NMAX, false, false>(xa, xb, xe);
+      else wg_gemm<double, 4, MAXT_SQ>(n, n, n, xa, xb, xe);
+      __syncthreads();
+      return 0;
+    }
   };
   for (int e = threadIdx.x; e < n0 * n0; e += 256) {
     const int i = e / n0, k = e % n0;
@@ -167,17 +230,18 @@ void rgf_transmission_kernel(RgfArgs a) {
   const double T = block_sum(acc, red);
   if (threadIdx.x == 0) {
     a.T[pt] = T;
-    if (a.status) a.status[pt] = sing ? STATUS_SINGULAR : 0;
+    if (a.status)
+      a.status[pt] = (sing ? STATUS_SINGULAR : 0) | (refine_fallback ? STATUS_REFINE_FALLBACK : 0);
   }
 }
 
-template <typename R, int NMAX, bool FIXED>
+template <typename R, int NMAX, bool FIXED, bool REFINE = false>
 static hipError_t launch_nmax(const RgfArgs& a, bool ortho, hipStream_t st) {
   dim3 grid(a.P), block(256);
   if (ortho)
-    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, true, FIXED>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, true, FIXED, REFINE>), grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, false, FIXED>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, false, FIXED, REFINE>), grid, block, 0, st, a);
   return hipGetLastError();
 }
 
@@ -203,6 +267,21 @@ template hipError_t launch_rgf_transmission<double>(const RgfArgs&, int, bool, b
 hipError_t launch_rgf_transmission_mixed(const RgfArgs& a, int nmax, bool uniform, bool ortho,
                                          hipStream_t st) {
   return launch_rgf_transmission<float>(a, nmax, uniform, ortho, st);
+}
+
+template <int NMAX>
+static hipError_t launch_refined_sel(const RgfArgs& a, bool uniform, int nmax, bool ortho,
+                                     hipStream_t st) {
+  if (uniform && nmax == NMAX) return launch_nmax<double, NMAX, true, true>(a, ortho, st);
+  return launch_nmax<double, NMAX, false, true>(a, ortho, st);
+}
+
+hipError_t launch_rgf_transmission_refined(const RgfArgs& a, int nmax, bool uniform, bool ortho,
+                                           hipStream_t st) {
+  if (nmax <= 16) return launch_refined_sel<16>(a, uniform, nmax, ortho, st);
+  if (nmax <= 32) return launch_refined_sel<32>(a, uniform, nmax, ortho, st);
+  if (nmax <= 48) return launch_refined_sel<48>(a, uniform, nmax, ortho, st);
+  return hipErrorInvalidValue;
 }
 
 }  // namespace negf

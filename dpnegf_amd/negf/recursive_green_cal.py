@@ -250,6 +250,10 @@ def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", sta
     """T for every point: ``z`` [P] complex (E + E_ref + i eta), ``sigL`` [P, nL, nL],
     ``sigR`` [P, nR, nR] (already trimmed to their corners), ``sys_idx`` [P] int32
     (k-point of each point).  Returns T [P] float64 (and fills ``status``).
+    ``precision``: "fp64"; "mixed" (FP32 throughout, about 1e-4 on T of order
+    30); or "mixed_refined" (blocks <= 48: FP32 block inverses refined by one
+    FP64 Newton step, FP64 products; ``status`` gets STATUS_REFINE_FALLBACK
+    where a block was inverted in FP64 instead).
     ``chunk``: points per call of the block-operation sweep (blocks above 64);
     by default ``autotune.default_chunk`` (DEFAULT_CHUNK, fewer when that many
     would not fit in half the free device memory), or
@@ -261,7 +265,8 @@ def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", sta
     P = z.shape[0]
     T = out if out is not None else torch.empty(P, dtype=torch.float64, device=dev)
     if max(systems.sizes) <= nat.TRANSMISSION_NMAX:
-        prec = {"fp64": nat.DPNEGF_FP64, "mixed": nat.DPNEGF_MIXED}[precision]
+        prec = {"fp64": nat.DPNEGF_FP64, "mixed": nat.DPNEGF_MIXED,
+                "mixed_refined": nat.DPNEGF_MIXED_REFINED}[precision]
         nat.call("dpnegf_rgf_transmission", nat.stream_ptr(dev), prec, len(systems.sizes),
                  systems._h_bsz, nat.ptr(systems.bsz), nat.ptr(systems.arena), systems.stride,
                  systems.nsys, nat.ptr(sys_idx), P, nat.ptr(z), nat.ptr(sigL), sigL.shape[-1],

@@ -30,7 +30,8 @@ extern "C" {
 #define DPNEGF_ERR_HIP (-3)
 
 #define DPNEGF_FP64 0
-#define DPNEGF_MIXED 1
+#define DPNEGF_MIXED 1         /* FP32 throughout (about 1e-4 on T of order 30) */
+#define DPNEGF_MIXED_REFINED 2 /* FP32 block inverses + one FP64 Newton step, FP64 products */
 
 /* size limits of the entry points */
 #define DPNEGF_TRANSMISSION_NMAX 64
@@ -60,7 +61,9 @@ int64_t dpnegf_arena_elems(int K, const int32_t* h_bsz);
  *   sys_idx[p] = system (k-point) of point p; arena + sys * sys_stride (null: all 0);
  *                an index outside [0, nsys) gives T[p] = NaN and DPNEGF_STATUS_BAD_INDEX
  *   ortho      = 1 if every sd = I and su = sl = 0 (S blocks are then not read)
- *   precision  = DPNEGF_FP64 or DPNEGF_MIXED
+ *   precision  = DPNEGF_FP64, DPNEGF_MIXED or DPNEGF_MIXED_REFINED (blocks <= 48;
+ *                a block whose FP32 inverse leaves a residual above 1e-4 is
+ *                inverted in FP64 instead: DPNEGF_STATUS_REFINE_FALLBACK)
  *   T[p]       = Re Tr[Gamma_L G_{0,K-1} Gamma_R G_{0,K-1}^dagger]
  *   status[p]  = DPNEGF_STATUS_* bits (may be null)
  * Block sizes up to DPNEGF_TRANSMISSION_NMAX. */

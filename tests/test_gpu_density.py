@@ -19,8 +19,11 @@ def _dev():
     return torch.device("cuda")
 
 
-def _oracle_density(s, kBT, mu_L, mu, R, M, E_ref, eta_lead, eta_dev, vL, vR, n_gauss, fL, fR):
-    poles, res = ooz.ozaki_residues(M)
+def _oracle_density(s, kBT, mu_L, mu, R, M, E_ref, eta_lead, eta_dev, vL, vR, n_gauss, fL, fR,
+                    poles=None, res=None):
+    """Oracle density; ``poles`` / ``res`` (optional) replace the M-pole set."""
+    if poles is None:
+        poles, res = ooz.ozaki_residues(M)
 
     def sig(e, eta):
         return [osg.selfEnergy(l[0], l[1], l[3], l[4], torch.tensor(e, dtype=torch.complex128),
