@@ -350,6 +350,104 @@ def bench_adjoint(args):
     }))
 
 
+class _RibbonHamiltonian:
+    """The headline ribbon behind the Hamiltonian-object calls the reference's
+    runner makes (``get_hs_device``, ``get_hs_lead``, ``device_norbs``;
+    ``negf_hamiltonian_init.py:745-974``)."""
+
+    def __init__(self, s):
+        self.s = s
+        self.device_norbs = [1] * sum(s.sizes)
+
+    def get_hs_device(self, kpoint, V=None, block_tridiagonal=True, only_subblocks=False):
+        if only_subblocks:
+            return None
+        s = self.s
+        return [list(s.hd), list(s.sd), list(s.hl), list(s.su), list(s.sl), list(s.hu)]
+
+    def get_hs_lead(self, kpoint, tab, v):
+        hL, hLL, hDL, sL, sLL, sDL = self.s.lead_L if tab == "lead_L" else self.s.lead_R
+        return hL - v * sL, hLL - v * sLL, hDL, sL, sLL, sDL
+
+
+def bench_dropin(args):
+    """The headline workload through the reference runner's unchanged loop
+    (``NEGF.py:750-816``) with the drop-in classes: ``compute_all_self_energy``
+    over the grid (``NEGF.py:558-598``; here one batched launch per lead),
+    then per chunk of energies ``LeadProperty.self_energy`` per energy (a
+    lookup of the stored values), ``cal_green_function`` on the chunk (every
+    retarded block) and ``_cal_tc_``."""
+    from dpnegf_amd.negf.device_property import DeviceProperty, transmission_pipeline
+    from dpnegf_amd.negf.lead_property import LeadProperty, compute_all_self_energy
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    eta = 1e-6
+    s_cpu, E = build_workload(dev, nE=args.energies)
+    s = s_cpu.to(dev)
+    ham = _RibbonHamiltonian(s)
+    leads = [LeadProperty(tab, ham, None, None, 0.0, e_T=300, efermi=0.0, device=dev)
+             for tab in ("lead_L", "lead_R")]
+    dp = DeviceProperty(ham, None, None, e_T=300, efermi=0.0, rgf_device=dev)
+    dp.set_leadLR(*leads)
+    k = [0.0, 0.0, 0.0]
+    legs = {}
+
+    def step():
+        t0 = time.perf_counter()
+        compute_all_self_energy(eta, leads[0], leads[1], [k], E)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        out, t_loop, t_gf, t_tc = [], 0.0, 0.0, 0.0
+        for e_chunk in torch.split(E, args.dropin_chunk):
+            ta = time.perf_counter()
+            seL, seR = [], []
+            for e in e_chunk:
+                for lead in leads:
+                    lead.self_energy(kpoint=k, energy=e, eta_lead=eta)
+                seL.append(leads[0].se)
+                seR.append(leads[1].se)
+            if len(e_chunk) > 1:
+                leads[0].se = torch.stack(seL, dim=0)
+                leads[1].se = torch.stack(seR, dim=0)
+            torch.cuda.synchronize()
+            tb = time.perf_counter()
+            dp.cal_green_function(energy=e_chunk, kpoint=k, eta_device=eta, block_tridiagonal=True,
+                                  Vbias=torch.tensor(0.0))
+            torch.cuda.synchronize()
+            tc_ = time.perf_counter()
+            out.append(dp.tc.reshape(-1).cpu())
+            dp.release_greenfuncs()
+            td = time.perf_counter()
+            t_loop, t_gf, t_tc = t_loop + tb - ta, t_gf + tc_ - tb, t_tc + td - tc_
+        legs.update(sigma_s=t1 - t0, se_lookup_s=t_loop, green_function_s=t_gf, tc_s=t_tc)
+        return torch.cat(out)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        T = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    T_fused = transmission_pipeline(s, E, eta, eta).cpu()
+    nE = E.shape[0]
+    print(json.dumps({
+        "metric": METRIC, "value": round(nE / dt, 3), "unit": "(E,k) points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * dt, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"drop-in route: ribbon W=48 K=128 Anderson U[-0.25,0.25] seed 0, "
+                               f"{nE} energies in [-3,3] eV, eta=1e-6, through the reference "
+                               f"runner's chunk loop (chunk {args.dropin_chunk}) with "
+                               f"dpnegf_amd LeadProperty / DeviceProperty",
+                   "points_per_step": nE},
+        "legs_s": {k_: round(v, 4) for k_, v in legs.items()},
+        "max_abs_dT_vs_fused_pipeline": float((T - T_fused).abs().max()),
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -360,10 +458,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="ribbon48",
-                    choices=["ribbon48", "multiorbital", "density", "adjoint"])
+                    choices=["ribbon48", "multiorbital", "density", "adjoint", "dropin"])
     ap.add_argument("--kpoints", type=int, default=None,
                     help="transverse k-points of the multiorbital (default 32) or density (default 1) "
                          "workload")
+    ap.add_argument("--dropin-chunk", type=int, default=1024,
+                    help="dropin: energies per cal_green_function call (e_batch_size)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -388,6 +488,9 @@ def main():
         return
     if args.workload == "adjoint":
         bench_adjoint(args)
+        return
+    if args.workload == "dropin":
+        bench_dropin(args)
         return
 
     from dpnegf_amd.negf.recursive_green_cal import BTDSystems, rgf_transmission

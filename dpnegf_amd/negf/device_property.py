@@ -32,6 +32,51 @@ def kBT_of(e_T):
     return _BOLTZMANN * e_T / _EV2J
 
 
+def _batched(x):
+    x = x.to(C128)
+    return (x if x.ndim == 3 else x.unsqueeze(0)).contiguous()
+
+
+def _mm(A, B, opA=0, opB=0):
+    """op(A) op(B) through the library's batched product
+    (``dpnegf_zgemm_batched``); 2-D operands are shared over the batch, a 2-D
+    result is returned when both are 2-D.  op 0 = N, 1 = conjugate transpose."""
+    squeeze = A.ndim == 2 and B.ndim == 2
+    Ab, Bb = _batched(A), _batched(B)
+    nb = max(Ab.shape[0], Bb.shape[0])
+    m, k = (Ab.shape[1], Ab.shape[2]) if opA == 0 else (Ab.shape[2], Ab.shape[1])
+    k2, n = (Bb.shape[1], Bb.shape[2]) if opB == 0 else (Bb.shape[2], Bb.shape[1])
+    assert k == k2, "inner dimensions differ"
+    C = torch.empty(nb, m, n, dtype=C128, device=Ab.device)
+    sa = 0 if Ab.shape[0] == 1 else Ab.shape[1] * Ab.shape[2]
+    sb = 0 if Bb.shape[0] == 1 else Bb.shape[1] * Bb.shape[2]
+    st = nat.stream_ptr(Ab.device)
+    for b0 in range(0, nb, 65535):
+        nbc = min(65535, nb - b0)
+        nat.call("dpnegf_zgemm_batched", st, nbc, m, n, k, nat.ptr(Ab[b0 * (sa > 0):]), Ab.shape[2],
+                 sa, opA, nat.ptr(Bb[b0 * (sb > 0):]), Bb.shape[2], sb, opB, 1.0, 0.0, 0.0, 0.0,
+                 nat.ptr(C[b0:]), n, m * n)
+    return C[0] if squeeze else C
+
+
+def _diag_of_product(pairs, batch, device):
+    """sum over (A, B) pairs of diag(A B) through ``dpnegf_diag_product``:
+    A [B, m, n] (or 2-D, shared), B [n, m] (or batched); returns [B, m]."""
+    out = None
+    st = nat.stream_ptr(device)
+    for i, (A, Bm) in enumerate(pairs):
+        Ab, Bb = _batched(A), _batched(Bm)
+        m, n = Ab.shape[1], Ab.shape[2]
+        assert Bb.shape[1] == n and Bb.shape[2] == m
+        if out is None:
+            out = torch.empty(batch, m, dtype=C128, device=device)
+        sa = 0 if Ab.shape[0] == 1 else m * n
+        sb = 0 if Bb.shape[0] == 1 else m * n
+        nat.call("dpnegf_diag_product", st, nat.ptr(out), nat.ptr(Ab), sa, nat.ptr(Bb), sb, m, n,
+                 batch, int(i > 0))
+    return out
+
+
 def _build_s_in_batched(hd, seinL, seinR, idx0, idy0, idx1, idy1):
     """``device_property.py:21-31``."""
     B = seinL.shape[0]
@@ -161,48 +206,53 @@ class DeviceProperty(object):
         torch.cuda.empty_cache()
 
     def _cal_tc_(self):
-        """``device_property.py:374-406``."""
+        """``device_property.py:374-406``: T = Re Tr[Gamma_L^pad G Gamma_R^pad G^dagger],
+        formed on the corner Q = G_{0,K-1}[:x0, -x1:] that the padded Gammas
+        select: V = Q Gamma_R, W = V Q^dagger, T = Re sum diag(Gamma_L W)
+        (library products and diagonal-of-product)."""
         g_trans = self.g_trans
         batched = g_trans.ndim == 3
         tx, ty = g_trans.shape[-2], g_trans.shape[-1]
         gL_full = self.lead_L.gamma.to(self.rgf_device)
         gR_full = self.lead_R.gamma.to(self.rgf_device)
         x0, x1 = min(gL_full.shape[-2], tx), min(gR_full.shape[-2], ty)
-        gl_shape = (g_trans.shape[0], tx, tx) if batched else (tx, tx)
-        gr_shape = (g_trans.shape[0], ty, ty) if batched else (ty, ty)
-        gL = torch.zeros(gl_shape, dtype=C128, device=self.rgf_device)
-        gR = torch.zeros(gr_shape, dtype=C128, device=self.rgf_device)
-        gL[..., :x0, :x0] += gL_full[..., :x0, :x0]
-        gR[..., -x1:, -x1:] += gR_full[..., -x1:, -x1:]
-        return (gL @ g_trans @ gR @ g_trans.mH).diagonal(dim1=-2, dim2=-1).real.sum(-1)
+        Q = g_trans[..., :x0, ty - x1:]
+        gLc = gL_full[..., :x0, :x0]
+        gRc = gR_full[..., gR_full.shape[-2] - x1:, gR_full.shape[-1] - x1:]
+        W = _mm(_mm(Q, gRc), Q, 0, 1)
+        nb = W.shape[0] if W.ndim == 3 else 1
+        T = _diag_of_product([(gLc, W)], nb, W.device).real.sum(-1)
+        return T if batched else T[0]
 
-    def _spectral_blocks(self):
-        out = []
+    def _spectral_diags(self):
+        """Per block, the diagonal of G S for the block row (``device_property.py:
+        428-438``): diag(G_qq S_qq) + diag(G_{q,q-1} S_{q-1,q}) + diag(G_{q,q+1}
+        S_{q+1,q}), through the library's diagonal-of-product kernel."""
         K = len(self.grd)
+        batched = self.grd[0].ndim == 3
+        nb = self.grd[0].shape[0] if batched else 1
+        out = []
         for jj in range(K):
-            if not self.block_tridiagonal or len(self.gru) == 0:
-                t = self.grd[jj] @ self.sd[jj]
-            elif jj == 0:
-                t = self.grd[jj] @ self.sd[jj] + self.gru[jj] @ self.sl[jj]
-            elif jj == K - 1:
-                t = self.grd[jj] @ self.sd[jj] + self.grl[jj - 1] @ self.su[jj - 1]
-            else:
-                t = (self.grd[jj] @ self.sd[jj] + self.grl[jj - 1] @ self.su[jj - 1]
-                     + self.gru[jj] @ self.sl[jj])
-            out.append(t)
+            pairs = [(self.grd[jj], self.sd[jj])]
+            if self.block_tridiagonal and len(self.gru) > 0:
+                if jj > 0:
+                    pairs.append((self.grl[jj - 1], self.su[jj - 1]))
+                if jj < K - 1:
+                    pairs.append((self.gru[jj], self.sl[jj]))
+            d = _diag_of_product(pairs, nb, self.rgf_device)
+            out.append(d if batched else d[0])
         return out
 
     def _cal_dos_(self):
         """``device_property.py:408-439``."""
         dos = 0
-        for t in self._spectral_blocks():
-            dos = dos - t.imag.diagonal(dim1=-2, dim2=-1).sum(-1) / math.pi
+        for d in self._spectral_diags():
+            dos = dos - d.imag.sum(-1) / math.pi
         return dos * 2
 
     def _cal_ldos_(self):
         """``device_property.py:441-470``."""
-        ldos = torch.cat([-t.imag.diagonal(dim1=-2, dim2=-1) / math.pi
-                          for t in self._spectral_blocks()], dim=-1).contiguous()
+        ldos = torch.cat([-d.imag / math.pi for d in self._spectral_diags()], dim=-1).contiguous()
         acc = np.cumsum([0] + list(self.norbs_per_atom))
         ldos = torch.stack([ldos[..., acc[i]:acc[i + 1]].sum(-1) for i in range(len(acc) - 1)],
                            dim=-1)

@@ -65,10 +65,47 @@ class LeadProperty(object):
 
     def self_energy(self, kpoint, energy, eta_lead=1e-5, method="Lopez-Sancho", save_path=None,
                     save_format="h5", se_info_display=False, HS_inmem=True):
-        """Sets ``self.se``: [a, a] for a scalar energy, [B, a, a] for a 1-D one."""
+        """Sets ``self.se``: [a, a] for a scalar energy, [B, a, a] for a 1-D one.
+
+        Load-or-compute as the reference (``lead_property.py:105-158``): a
+        self-energy stored by :func:`compute_all_self_energy` for this k-point
+        and energy (the key of the reference's files, ``E_{e:.8f}`` and the
+        k-point) is returned without a launch; otherwise it is computed."""
         assert len(np.array(kpoint).reshape(-1)) == 3
+        hit = self._stored(kpoint, energy)
+        if hit is not None:
+            self.se = hit
+            return
         self.se = self.self_energy_cal(kpoint, energy, eta_lead=eta_lead, method=method,
                                        HS_inmem=HS_inmem)
+
+    def store_self_energies(self, kpoints, energies, eta_lead):
+        """Self-energies of every (k, E) in one batched launch per k-point,
+        kept on the device for :meth:`self_energy` to read (in place of the
+        reference's per-(k, E) HDF5 records, ``lead_property.py:932-951``)."""
+        E = torch.as_tensor(energies).reshape(-1)
+        keys = [f"{float(e.real):.8f}" for e in E]
+        if not hasattr(self, "_se_store"):
+            self._se_store = {}
+        for k in kpoints:
+            sig = self.sigma_fn(k)(E, eta_lead)
+            self._se_store[_k_key(k)] = ({key: i for i, key in enumerate(keys)}, sig)
+
+    def _stored(self, kpoint, energy):
+        store = getattr(self, "_se_store", None)
+        if not store:
+            return None
+        ent = store.get(_k_key(kpoint))
+        if ent is None:
+            return None
+        e = torch.as_tensor(energy)
+        if e.ndim == 0:
+            i = ent[0].get(f"{float(e.real):.8f}")
+            return None if i is None else ent[1][i]
+        idx = [ent[0].get(f"{float(x.real):.8f}") for x in e.reshape(-1)]
+        if any(i is None for i in idx):
+            return None
+        return ent[1][torch.as_tensor(idx, device=ent[1].device)]
 
     def self_energy_cal(self, kpoint, energy, eta_lead=1e-5, method="Lopez-Sancho",
                         se_numba_jit=None, HS_inmem=True):
@@ -146,6 +183,28 @@ class LeadProperty(object):
     @property
     def gamma(self):
         return self.sigmaLR2Gamma(self.se)
+
+
+def _k_key(k):
+    """A k-point as a hashable key (the reference keys its records the same way)."""
+    a = np.asarray(torch.as_tensor(k).detach().cpu().numpy() if torch.is_tensor(k) else k,
+                   dtype=float).reshape(3)
+    return (float(a[0]), float(a[1]), float(a[2]))
+
+
+def compute_all_self_energy(eta, lead_L, lead_R, kpoints_grid, energy_grid,
+                            self_energy_save_path=None, n_jobs=-1, batch_size=200, n_cpus=None,
+                            se_numba_jit=None):
+    """``lead_property.py:599-684`` on the GPU: the self-energies of both
+    leads at every (k, E) of the grid, one batched decimation launch per lead
+    and k-point instead of a loky pool over (k, E) tasks, kept in device
+    memory on the lead objects instead of HDF5 files.  The runner's loop then
+    reads them through ``LeadProperty.self_energy`` as it reads the files
+    (``NEGF.py:762-774``).  ``self_energy_save_path``, ``n_jobs``,
+    ``batch_size``, ``n_cpus`` and ``se_numba_jit`` are accepted for the
+    reference's signature and not needed."""
+    for lead in (lead_L, lead_R):
+        lead.store_self_energies(kpoints_grid, energy_grid, eta)
 
 
 def bloch_self_energy(small, k_bloch, R_list, sorted_idx, hDL, sDL, energies, eta_lead,
