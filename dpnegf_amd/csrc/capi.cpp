@@ -280,7 +280,8 @@ int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena
                     const void* z, const void* sigL, int nL, const void* sigR, int nR, int ortho,
                     const void* seP, void* gl, void* grd, void* grl, void* gru, void* lc, void* g_trans,
                     const void* s_in, void* gin, void* gnd, void* gnl, void* gnu,
-                    const void* s_out, void* gip, void* gpd, void* gpl, void* gpu, void* work,
+                    const void* s_out, void* gip, void* gpd, void* gpl, void* gpu,
+                    const void* s_bil, void* gbi, void* gbd, void* gbl, void* gbu, void* work,
                     int32_t* status) {
   int nmax = 0;
   if (int r = check_blocks(K, h_bsz, DPNEGF_FULL_NMAX, &nmax)) return r;
@@ -296,6 +297,8 @@ int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena
     return fail(DPNEGF_ERR_ARG, "lesser outputs missing");
   if (s_out && (!gip || !gpd || (K > 1 && (!gpl || !gpu))))
     return fail(DPNEGF_ERR_ARG, "greater outputs missing");
+  if (s_bil && (!gbi || !gbd || (K > 1 && (!gbl || !gbu))))
+    return fail(DPNEGF_ERR_ARG, "bilinear outputs missing");
   negf::RgfFullArgs a;
   std::memset(&a, 0, sizeof(a));
   a.K = K; a.B = B; a.nmax = nmax; a.ortho = ortho; a.bsz = h_bsz;
@@ -308,6 +311,8 @@ int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena
   a.gnl = (cx<double>*)gnl; a.gnu = (cx<double>*)gnu;
   a.s_out = (const cx<double>*)s_out; a.gip = (cx<double>*)gip; a.gpd = (cx<double>*)gpd;
   a.gpl = (cx<double>*)gpl; a.gpu = (cx<double>*)gpu;
+  a.s_bil = (const cx<double>*)s_bil; a.gbi = (cx<double>*)gbi; a.gbd = (cx<double>*)gbd;
+  a.gbl = (cx<double>*)gbl; a.gbu = (cx<double>*)gbu;
   a.work = (cx<double>*)work; a.status = status;
   a.inv_ws = nmax > negf::GJ_NMAX_LARGE ? a.work + 5 * (int64_t)B * nmax * nmax : nullptr;
   hipError_t e = negf::rgf_full(a, (hipStream_t)stream);
@@ -374,6 +379,17 @@ int dpnegf_rgf_transmission_grad_ckpt(void* stream, int K, const int32_t* h_bsz,
   if (r) return r < 0 ? r : 0;
   hipError_t e = negf::rgf_grad_ckpt(a, stride, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "rgf_transmission_grad_ckpt");
+  return 0;
+}
+
+int dpnegf_batch_sum_conjt(void* stream, void* out, const void* in, int m, int n, int B,
+                           int accumulate) {
+  if (m < 0 || n < 0 || B < 0) return fail(DPNEGF_ERR_ARG, "negative size");
+  if (m == 0 || n == 0) return 0;
+  if (!out || (B > 0 && !in)) return fail(DPNEGF_ERR_ARG, "null device pointer");
+  hipError_t e = negf::batch_sum_conjt((cx<double>*)out, (const cx<double>*)in, m, n, B, accumulate,
+                                       (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "batch_sum_conjt");
   return 0;
 }
 

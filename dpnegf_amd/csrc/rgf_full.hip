@@ -307,7 +307,7 @@ __global__ void badd_identity_kernel(cx<double>* D, int n, int B) {
 // workgroups.  The partial sums of an element are added in a fixed order.
 constexpr int RED_ELEMS = 16, RED_PHASES = 16;
 __global__ __launch_bounds__(256) void breduce_conjt_kernel(cx<double>* out, const cx<double>* in,
-                                                            int m, int n, int B) {
+                                                            int m, int n, int B, int accumulate) {
   __shared__ cx<double> part[RED_PHASES][RED_ELEMS];
   const int64_t mn = (int64_t)m * n;
   const int el = threadIdx.x % RED_ELEMS, ph = threadIdx.x / RED_ELEMS;
@@ -321,7 +321,8 @@ __global__ __launch_bounds__(256) void breduce_conjt_kernel(cx<double>* out, con
     cx<double> sum = part[0][el];
     for (int p = 1; p < RED_PHASES; ++p) sum = sum + part[p][el];
     const int k = (int)(e / n), i = (int)(e % n);
-    out[(int64_t)i * m + k] = conj(sum);
+    const int64_t o = (int64_t)i * m + k;
+    out[o] = accumulate ? out[o] + conj(sum) : conj(sum);
   }
 }
 
@@ -427,6 +428,14 @@ hipError_t weighted_re_sum(double* out, const cx<double>* in, const cx<double>* 
   return hipGetLastError();
 }
 
+
+hipError_t batch_sum_conjt(cx<double>* out, const cx<double>* in, int m, int n, int B,
+                           int accumulate, hipStream_t st) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(breduce_conjt_kernel, dim3(((int64_t)m * n + RED_ELEMS - 1) / RED_ELEMS),
+                     dim3(256), 0, st, out, in, m, n, B, accumulate);
+  return hipGetLastError();
+}
 
 // Lopez-Sancho helpers for the batched (large-block) path -----------------------
 // alpha0 = h10 - z s10, beta0 = h01 - z s01, eps = eps_s = H, per point
@@ -831,16 +840,21 @@ hipError_t rgf_full(const RgfFullArgs& a, hipStream_t st) {
   }
   CHK(hipMemcpyAsync(a.g_trans, col, sizeof(cx<double>) * B * n[0] * nK, hipMemcpyDeviceToDevice, st));
 
-  // ---- lesser (:241-264) and greater (:267-289)
-  for (int pass = 0; pass < 2; ++pass) {
-    const cx<double>* sin = pass == 0 ? a.s_in : a.s_out;
-    cx<double>* gin = pass == 0 ? a.gin : a.gip;
-    cx<double>* gnd = pass == 0 ? a.gnd : a.gpd;
-    cx<double>* gnl = pass == 0 ? a.gnl : a.gpl;
-    cx<double>* gnu = pass == 0 ? a.gnu : a.gpu;
+  // ---- lesser (:241-264), greater (:267-289) and the bilinear G S G (pass 2,
+  // no counterpart in the reference: the density adjoint's G W G, with
+  // x_{q+1} = g (S + L x_q U) g forward and, backward,
+  //   X_{q+1,q} = G_{q+1,q+1} L x_q + X_{q+1,q+1} L g_q
+  //   X_{q,q+1} = x_q U G_{q+1,q+1} + g_q U X_{q+1,q+1}
+  //   X_qq = x_q + G_{q,q+1} L x_q + x_q U G_{q+1,q} + g_q U X_{q+1,q+1} L g_q)
+  for (int pass = 0; pass < 3; ++pass) {
+    const cx<double>* sin = pass == 0 ? a.s_in : pass == 1 ? a.s_out : a.s_bil;
+    cx<double>* gin = pass == 0 ? a.gin : pass == 1 ? a.gip : a.gbi;
+    cx<double>* gnd = pass == 0 ? a.gnd : pass == 1 ? a.gpd : a.gbd;
+    cx<double>* gnl = pass == 0 ? a.gnl : pass == 1 ? a.gpl : a.gbl;
+    cx<double>* gnu = pass == 0 ? a.gnu : pass == 1 ? a.gpu : a.gbu;
     if (!sin) continue;
-    // lesser conjugates with ^H, the greater quirk with plain conj
-    const int cj = pass == 0 ? OP_H : OP_C;
+    // lesser conjugates with ^H, the greater quirk with plain conj, bilinear not at all
+    const int cj = pass == 0 ? OP_H : pass == 1 ? OP_C : OP_N;
     auto gi = [&](int q) { return gin + od[q]; };
     auto si = [&](int q) { return sin + od[q]; };
     // gin_0 = g_0 s_0 g_0^(cj)
@@ -854,7 +868,7 @@ hipError_t rgf_full(const RgfFullArgs& a, hipStream_t st) {
       CHK(gemm(st, B, n1, nq, nq, Lop(q), mat(gi(q), nq, (int64_t)nq * nq), one, zero, w0,
                (int64_t)n1 * nq, nq, z));
       CHK(hipMemcpyAsync(w1, si(q + 1), sizeof(cx<double>) * B * n1 * n1, hipMemcpyDeviceToDevice, st));
-      CHK(gemm(st, B, n1, n1, nq, mat(w0, nq, (int64_t)n1 * nq), Uop(q, pass == 0 ? OP_N : OP_C), one,
+      CHK(gemm(st, B, n1, n1, nq, mat(w0, nq, (int64_t)n1 * nq), Uop(q, pass == 1 ? OP_C : OP_N), one,
                one, w1, (int64_t)n1 * n1, n1, z));
       // gin_{q+1} = g src g^(cj)
       CHK(gemm(st, B, n1, n1, n1, mat(gl(q + 1), n1, (int64_t)n1 * n1), mat(w1, n1, (int64_t)n1 * n1),
@@ -871,7 +885,7 @@ hipError_t rgf_full(const RgfFullArgs& a, hipStream_t st) {
       cx<double>* nd = gnd + od[q];
       const cx<double>* nd1 = gnd + od[q + 1];
       // lg = L^(c) g_q^(cj)  [n1 x nq]  (lesser: L g^H ; greater: conj(L) conj(g))
-      CHK(gemm(st, B, n1, nq, nq, Lop(q, pass == 0 ? OP_N : OP_C), mat(gl(q), nq, sq, cj), one, zero,
+      CHK(gemm(st, B, n1, nq, nq, Lop(q, pass == 1 ? OP_C : OP_N), mat(gl(q), nq, sq, cj), one, zero,
                w0, (int64_t)n1 * nq, nq, z));
       // gnl = (G_{q+1,q+1} L) gin_q + gnd_{q+1} lg
       CHK(gemm(st, B, n1, nq, n1, mat(grd(q + 1), n1, s1), Lop(q), one, zero, w1, (int64_t)n1 * nq,
@@ -885,17 +899,20 @@ hipError_t rgf_full(const RgfFullArgs& a, hipStream_t st) {
       CHK(gemm(st, B, nq, n1, nq, mat(gl(q), nq, sq), Uop(q), one, zero, w1, sx, n1, z));
       CHK(gemm(st, B, nq, n1, n1, mat(w1, n1, sx), mat(nd1, n1, s1), one, zero, w2, sx, n1, z));
       CHK(gemm(st, B, nq, nq, n1, mat(w2, n1, sx), mat(w0, nq, (int64_t)n1 * nq), one, one, nd, sq, nq, z));
-      CHK(gemm(st, B, nq, n1, nq, mat(gi(q), nq, sq), Uop(q, pass == 0 ? OP_N : OP_C), one, zero, w1,
+      CHK(gemm(st, B, nq, n1, nq, mat(gi(q), nq, sq), Uop(q, pass == 1 ? OP_C : OP_N), one, zero, w1,
                sx, n1, z));
       if (pass == 0)  // ... gru[q]^H  (n1 x nq operand from gru stored nq x n1)
         CHK(gemm(st, B, nq, nq, n1, mat(w1, n1, sx), mat(gru(q), n1, sx, OP_H), one, one, nd, sq, nq, z));
-      else            // ... conj(grl[q])  (grl stored n1 x nq)
-        CHK(gemm(st, B, nq, nq, n1, mat(w1, n1, sx), mat(grl(q), nq, (int64_t)n1 * nq, OP_C), one, one,
-                 nd, sq, nq, z));
+      else            // ... conj(grl[q]) (greater) or grl[q] (bilinear)  (grl stored n1 x nq)
+        CHK(gemm(st, B, nq, nq, n1, mat(w1, n1, sx),
+                 mat(grl(q), nq, (int64_t)n1 * nq, pass == 1 ? OP_C : OP_N), one, one, nd, sq, nq, z));
+      if (pass == 2)  // X_{q,q+1} = (x_q U) G_{q+1,q+1} + (g_q U) X_{q+1,q+1}  (the latter in w2)
+        CHK(gemm(st, B, nq, n1, n1, mat(w1, n1, sx), mat(grd(q + 1), n1, s1), one, one, gnu + ou[q], sx,
+                 n1, z, w2));
       CHK(gemm(st, B, nq, nq, n1, mat(gru(q), n1, sx), Lop(q), one, zero, w1, sq, nq, z));
       CHK(gemm(st, B, nq, nq, nq, mat(w1, nq, sq), mat(gi(q), nq, sq), one, one, nd, sq, nq, z));
-      // gnu = gnl^H
-      {
+      // gnu = gnl^H (lesser and greater)
+      if (pass < 2) {
         const int64_t tot = (int64_t)B * nq * n1;
         const int grid = (int)std::min<int64_t>((tot + 255) / 256, 65535);
         hipLaunchKernelGGL(bconjt_kernel, dim3(grid), dim3(256), 0, st, gnu + ou[q], nl, n1, nq, B);
@@ -1212,7 +1229,7 @@ hipError_t rgf_grad(const RgfGradArgs& g, hipStream_t st) {
   // first block row: G_{0,0} = grd_0; G_{0,b} = R_b G_bb, R_1 = g_0 U_0, R_{b+1} = R_b g_b U_b
   auto reduce = [&](cx<double>* out, const cx<double>* in, int m_, int n_) {
     hipLaunchKernelGGL(breduce_conjt_kernel, dim3((m_ * n_ + RED_ELEMS - 1) / RED_ELEMS), dim3(256),
-                       0, st, out, in, m_, n_, B);
+                       0, st, out, in, m_, n_, B, 0);
     return hipGetLastError();
   };
   const cx<double>* g0b = grd(0);  // G_{0,0}
@@ -1380,7 +1397,7 @@ hipError_t rgf_grad_ckpt(const RgfGradArgs& g, int stride, hipStream_t st) {
   };
   auto reduce = [&](cx<double>* out, const cx<double>* in, int m_, int n_) {
     hipLaunchKernelGGL(breduce_conjt_kernel, dim3((m_ * n_ + RED_ELEMS - 1) / RED_ELEMS), dim3(256),
-                       0, st, out, in, m_, n_, B);
+                       0, st, out, in, m_, n_, B, 0);
     return hipGetLastError();
   };
 

@@ -54,6 +54,8 @@ struct RgfFullArgs {
   cx<double>* gin; cx<double>* gnd; cx<double>* gnl; cx<double>* gnu;
   const cx<double>* s_out;     // greater source or null
   cx<double>* gip; cx<double>* gpd; cx<double>* gpl; cx<double>* gpu;
+  const cx<double>* s_bil;     // bilinear G S G source or null
+  cx<double>* gbi; cx<double>* gbd; cx<double>* gbl; cx<double>* gbu;
   cx<double>* work;            // 5 * B * nmax^2
   cx<double>* inv_ws;          // inverse_workspace(nmax, B) (null when nmax <= GJ_NMAX_LARGE)
   int32_t* status;             // [B] or null
@@ -87,6 +89,9 @@ hipError_t zgemm_batched(int B, int m, int n, int k, const cx<double>* A, int ld
                          int opA, const cx<double>* Bm, int ldb, int64_t sb, int opB,
                          cx<double> alpha, cx<double> beta, cx<double>* C, int ldc, int64_t sc,
                          hipStream_t st);
+// out (+)= (sum_b in[b])^H for B contiguous m x n blocks; out is n x m.
+hipError_t batch_sum_conjt(cx<double>* out, const cx<double>* in, int m, int n, int B,
+                           int accumulate, hipStream_t st);
 hipError_t weighted_re_sum(double* out, const cx<double>* in, const cx<double>* c, int64_t count,
                            int B, int accumulate, hipStream_t st);
 int64_t rgf_grad_workspace(int K, const int32_t* bsz, int B);

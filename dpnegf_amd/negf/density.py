@@ -73,25 +73,21 @@ def _weighted_blocks(arena, B, shapes, coef, acc=None):
     return out
 
 
-def ozaki_density(blocks, lead_L, lead_R, kBT, mu_L, mu, R, poles, residues, E_ref, eta_lead,
-                  eta_device, vL=0.0, vR=0.0, n_gauss=10, fermi_L=None, fermi_R=None,
-                  eta_lead0=1e-5, eta_device0=0.0):
-    """Density matrices of every diagonal block of one k-point.
+def ozaki_points(sizes, lead_L, lead_R, kBT, mu_L, mu, R, poles, residues, E_ref, eta_lead,
+                 eta_device, vL=0.0, vR=0.0, n_gauss=10, fermi_L=None, fermi_R=None,
+                 eta_lead0=1e-5, eta_device0=0.0, device=None):
+    """The energy points of the Ozaki density (``density.py:164-195``) with
+    their self-energies and weights.  Returns ``(eq, neq)``:
 
-    ``blocks`` = (hd, hu, hl, sd, su, sl) CUDA tensors; ``lead_X`` = lead tuples
-    (hL, hLL, hDL, sL, sLL, sDL), or callables ``lead_X(e, eta)`` returning the
-    self-energies [B, a, a] at energies ``e``.  Energies are relative to ``E_ref``.
-    Returns (DM_eq list, DM_neq list or 0.)."""
-    hd, hu, hl, sd, su, sl = blocks
-    dev = hd[0].device
-    nat.require_device(hd[0])
-    sizes = block_sizes(hd)
-    arena = pack_arena(hd, hu, hl, sd, su, sl, dev)
-    ortho = is_orthogonal(sd, su, sl)
+    * ``eq`` = (z [M+1], sigL, nL, sigR, nR, coef [M+1]): the zeroth moment
+      at iR - mu (broadening eta_lead0 / eta_device0) and the poles, with
+      DM_eq = sum_b Re(coef_b G_qq(z_b));
+    * ``neq`` = (z [n_gauss], sigL, nL, sigR, nR, gammaR [n_gauss, nK, nK],
+      a [n_gauss]) for a bias window (None at zero bias), with
+      DM_neq = sum_i a_i Re[G_{q,K-1} Gamma_R G_{q,K-1}^dagger](x_i)."""
+    dev = device
     poles = torch.as_tensor(poles, dtype=torch.float64).to(dev)
     residues = torch.as_tensor(residues, dtype=torch.float64).to(dev)
-    M = poles.shape[0]
-    # zeroth moment (density.py:167-172) + poles (:166, :173-179)
     e0 = torch.tensor([1j * R - mu], dtype=C128, device=dev)
     ep = (1j * poles * kBT + (mu_L - mu)).to(C128)
     s0 = _sigmas(lead_L, lead_R, e0, eta_lead0, E_ref, sizes[0], sizes[-1])
@@ -101,20 +97,9 @@ def ozaki_density(blocks, lead_L, lead_R, kBT, mu_L, mu, R, poles, residues, E_r
     z = torch.cat([e0 + E_ref + 1j * eta_device0, ep + E_ref + 1j * eta_device]).contiguous()
     coef = torch.cat([torch.tensor([1j * R], dtype=C128, device=dev),
                       (4.0 * kBT * residues).to(C128)]).contiguous()
-    # all points in one call unless their outputs would not fit (blocks of
-    # several hundred orbitals): then in chunks, summed in order
-    dm_eq = None
-    chunk = full_points_per_call(sizes, False, M + 1, dev)
-    for c0 in range(0, M + 1, chunk):
-        c = slice(c0, min(M + 1, c0 + chunk))
-        raw = _run_full(z[c].contiguous(), arena, sizes, ortho, sL[c].contiguous(), s0[1],
-                        sR[c].contiguous(), s0[3], False)
-        dm_eq = _weighted_blocks(raw["grd"], z[c].shape[0], [(n, n) for n in sizes],
-                                 coef[c].contiguous(), dm_eq)
-        del raw
+    eq = (z, sL, s0[1], sR, s0[3], coef)
     if abs(vL - vR) <= 1e-14:
-        return dm_eq, 0.
-    # non-equilibrium window (density.py:183-195)
+        return eq, None
     xl, xu = min(vL, vR) - 8 * kBT, max(vL, vR) + 8 * kBT
     xs, wl = gauss_xw(torch.scalar_tensor(xl, dtype=torch.float64),
                       torch.scalar_tensor(xu, dtype=torch.float64), n=n_gauss)
@@ -128,8 +113,36 @@ def ozaki_density(blocks, lead_L, lead_R, kBT, mu_L, mu, R, poles, residues, E_r
     gamR[:, nK - nR:, nK - nR:] = gam
     xsd = xs.to(dev)
     f = fermi_R(xsd + mu) - fermi_L(xsd + mu)
-    coef = (wl.to(dev) * f).to(C128).contiguous()
-    dm_neq = [None] * len(sizes)
+    a = (wl.to(dev) * f).to(torch.float64).contiguous()
+    return eq, (zx, sLx, nL, sRx, nR, gamR.contiguous(), a)
+
+
+def eq_density(arena, sizes, ortho, eq):
+    """DM_eq blocks of the equilibrium point set (see :func:`ozaki_points`);
+    all points in one call unless their outputs would not fit (blocks of
+    several hundred orbitals): then in chunks, summed in order."""
+    z, sL, nL, sR, nR, coef = eq
+    P = z.shape[0]
+    dm = None
+    chunk = full_points_per_call(sizes, False, P, z.device)
+    for c0 in range(0, P, chunk):
+        c = slice(c0, min(P, c0 + chunk))
+        raw = _run_full(z[c].contiguous(), arena, sizes, ortho, sL[c].contiguous(), nL,
+                        sR[c].contiguous(), nR, False)
+        dm = _weighted_blocks(raw["grd"], z[c].shape[0], [(n, n) for n in sizes],
+                              coef[c].contiguous(), dm)
+        del raw
+    return dm
+
+
+def neq_density(arena, sizes, ortho, neq):
+    """DM_neq blocks of the bias-window point set (see :func:`ozaki_points`)."""
+    zx, sLx, nL, sRx, nR, gamR, a = neq
+    dev = zx.device
+    G = zx.shape[0]
+    nK = sizes[-1]
+    coef = a.to(C128).contiguous()
+    dm = [None] * len(sizes)
     chunk = full_points_per_call(sizes, True, G, dev)
     for c0 in range(0, G, chunk):
         c = slice(c0, min(G, c0 + chunk))
@@ -148,10 +161,34 @@ def ozaki_density(blocks, lead_L, lead_R, kBT, mu_L, mu, R, poles, residues, E_r
                      nat.ptr(gR), nK, nK * nK, 0, 1.0, 0.0, 0.0, 0.0, nat.ptr(X), nK, n * nK)
             nat.call("dpnegf_zgemm_batched", st, Gc, n, n, nK, nat.ptr(X), nK, n * nK, 0,
                      nat.ptr(lcq), nK, n * nK, 1, 1.0, 0.0, 0.0, 0.0, nat.ptr(Y), n, n * n)
-            acc = [dm_neq[q]] if dm_neq[q] is not None else None
-            dm_neq[q] = _weighted_blocks(Y.reshape(-1), Gc, [(n, n)], cc, acc)[0]
+            acc = [dm[q]] if dm[q] is not None else None
+            dm[q] = _weighted_blocks(Y.reshape(-1), Gc, [(n, n)], cc, acc)[0]
         del raw
-    return dm_eq, dm_neq
+    return dm
+
+
+def ozaki_density(blocks, lead_L, lead_R, kBT, mu_L, mu, R, poles, residues, E_ref, eta_lead,
+                  eta_device, vL=0.0, vR=0.0, n_gauss=10, fermi_L=None, fermi_R=None,
+                  eta_lead0=1e-5, eta_device0=0.0):
+    """Density matrices of every diagonal block of one k-point.
+
+    ``blocks`` = (hd, hu, hl, sd, su, sl) CUDA tensors; ``lead_X`` = lead tuples
+    (hL, hLL, hDL, sL, sLL, sDL), or callables ``lead_X(e, eta)`` returning the
+    self-energies [B, a, a] at energies ``e``.  Energies are relative to ``E_ref``.
+    Returns (DM_eq list, DM_neq list or 0.)."""
+    hd, hu, hl, sd, su, sl = blocks
+    dev = hd[0].device
+    nat.require_device(hd[0])
+    sizes = block_sizes(hd)
+    arena = pack_arena(hd, hu, hl, sd, su, sl, dev)
+    ortho = is_orthogonal(sd, su, sl)
+    eq, neq = ozaki_points(sizes, lead_L, lead_R, kBT, mu_L, mu, R, poles, residues, E_ref,
+                           eta_lead, eta_device, vL, vR, n_gauss, fermi_L, fermi_R, eta_lead0,
+                           eta_device0, dev)
+    dm_eq = eq_density(arena, sizes, ortho, eq)
+    if neq is None:
+        return dm_eq, 0.
+    return dm_eq, neq_density(arena, sizes, ortho, neq)
 
 
 class Ozaki(object):

@@ -148,9 +148,11 @@ def full_points_per_call(sizes, need_lc, P, device, mem_fraction=0.5):
 
 
 def _run_full(z, arena, sizes, ortho, sigL, nL, sigR, nR, need_lc, s_in=None, s_out=None,
-              seP=None):
+              seP=None, s_bil=None):
     """One ``dpnegf_rgf_full`` call; returns the raw block-major arenas.
-    ``seP``: optional per-energy diagonal addend, a block-major [B, n_q, n_q] arena."""
+    ``seP``: optional per-energy diagonal addend, a block-major [B, n_q, n_q] arena.
+    ``s_bil``: optional block-diagonal source of the bilinear product G S G
+    (``out["bilinear"]`` = (s_bil, x_left, X_qq, X_{q+1,q}, X_{q,q+1}))."""
     device = z.device
     B = z.shape[0]
     K = len(sizes)
@@ -171,6 +173,8 @@ def _run_full(z, arena, sizes, ortho, sigL, nL, sigR, nR, need_lc, s_in=None, s_
                      if s_in is not None else None)
     out["greater"] = ((s_out, alloc(nsq), alloc(nsq), alloc(nx), alloc(nx))
                       if s_out is not None else None)
+    out["bilinear"] = ((s_bil, alloc(nsq), alloc(nsq), alloc(nx), alloc(nx))
+                       if s_bil is not None else None)
     status = torch.zeros(B, dtype=torch.int32, device=device)
     # the library's own size: it includes the block inverse's workspace for blocks above 128
     nwork = nat.load().dpnegf_rgf_full_workspace_elems(K, nat.host_i32(sizes), B)
@@ -181,7 +185,7 @@ def _run_full(z, arena, sizes, ortho, sigL, nL, sigR, nR, need_lc, s_in=None, s_
             nat.ptr(sigL), nL, nat.ptr(sigR), nR, int(ortho), nat.ptr(seP),
             nat.ptr(out["gl"]), nat.ptr(out["grd"]), nat.ptr(out["grl"]), nat.ptr(out["gru"]),
             nat.ptr(out["lc"]), nat.ptr(out["g_trans"])]
-    for part in (out["lesser"], out["greater"]):
+    for part in (out["lesser"], out["greater"], out["bilinear"]):
         args += [nat.ptr(t) for t in part] if part else [nat.ptr(None)] * 5
     args += [nat.ptr(work), nat.ptr(status)]
     nat.call("dpnegf_rgf_full", *args)

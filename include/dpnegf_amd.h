@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define DPNEGF_ABI_VERSION 2
+#define DPNEGF_ABI_VERSION 3
 
 #define DPNEGF_OK 0
 #define DPNEGF_ERR_ARG (-1)
@@ -158,13 +158,25 @@ int64_t dpnegf_rgf_full_workspace_elems(int K, const int32_t* h_bsz, int B);
  *   s_in / s_out (may be null: no lesser / greater): B x n_q x n_q arenas
  *   seP (may be null): energy-dependent addend of every diagonal block,
  *                 B x n_q x n_q arena (the 3-D seP of recursive_gf, :411-414)
+ *   s_bil (may be null): block-diagonal source S of the bilinear product
+ *                 X = G S G (no counterpart in the reference: the adjoint of the
+ *                 density, dJ/dH = (sum_b c_b G W G)^H); its blocks go to
+ *                 gbi (left-connected), gbd, gbl (X_{q+1,q}), gbu (X_{q,q+1})
  * Block sizes up to DPNEGF_FULL_NMAX; B <= 65535 per call. */
 int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena, int B,
                     const void* z, const void* sigL, int nL, const void* sigR, int nR, int ortho,
                     const void* seP, void* gl, void* grd, void* grl, void* gru, void* lc, void* g_trans,
                     const void* s_in, void* gin, void* gnd, void* gnl, void* gnu,
-                    const void* s_out, void* gip, void* gpd, void* gpl, void* gpu, void* work,
+                    const void* s_out, void* gip, void* gpd, void* gpl, void* gpu,
+                    const void* s_bil, void* gbi, void* gbd, void* gbl, void* gbu, void* work,
                     int32_t* status);
+
+/* out (+)= (sum_b in[b])^H: B contiguous m x n complex blocks summed over the
+ * batch, conjugated and transposed into out [n x m] (a gradient block in
+ * torch's complex convention, dJ/dRe + i dJ/dIm).  Partial sums are added in
+ * a fixed order.  No counterpart in the reference (autograd does this). */
+int dpnegf_batch_sum_conjt(void* stream, void* out, const void* in, int m, int n, int B,
+                           int accumulate);
 
 /* Complex elements of the work buffer dpnegf_rgf_transmission_grad needs. */
 int64_t dpnegf_rgf_grad_workspace_elems(int K, const int32_t* h_bsz, int B);

@@ -44,7 +44,7 @@ def test_argument_errors_are_reported_without_a_device():
     assert rc == -1
     with pytest.raises(ValueError):
         nat.call("dpnegf_rgf_full", None, 0, None, None, 1, None, None, 0, None, 0, 1,
-                 *([None] * 18))
+                 *([None] * 23))
     for n in (0, 129):
         assert lib.dpnegf_zinv_batched(None, None, n, 1, None) < 0
         assert b"outside" in lib.dpnegf_last_error()
