@@ -18,11 +18,14 @@ efermi - (V_L + V_R)/2 (the expression of ``device_property.py:134``).  The
 non-equilibrium term uses G_{q,K-1} Gamma_R G_{q,K-1}^dagger, which is the
 reference's G_00 Gamma_R G_00^dagger for a single-block device.
 """
+import math
+
 import numpy as np
 import torch
 
 from .. import _native as nat
 from ._btd import C128, block_sizes, corner_self_energy, is_orthogonal, pack_arena, split_blocks
+from ._ops import diag_of_product, mm
 from .ozaki_res_cal import ozaki_residues
 from .recursive_green_cal import _run_full, full_points_per_call
 from .surface_green import self_energy_batched
@@ -230,3 +233,154 @@ class Ozaki(object):
         onsite = torch.stack([DM[norbs[i]:norbs[i + 1]].sum() for i in range(len(norbs) - 1)])
         pos = torch.as_tensor(np.asarray(deviceprop.structure.positions)).to(onsite)
         return torch.cat([pos, onsite.unsqueeze(-1)], dim=-1)
+
+
+def _lead_sigmas(lead, kpoint, E, eta):
+    """Self-energies [B, a, a] of ``lead`` at the energies ``E``: values stored
+    by ``compute_all_self_energy`` when present (the reference's Dirichlet-lead
+    branch loads them), else one batched decimation."""
+    stored = lead._stored(kpoint, E) if hasattr(lead, "_stored") else None
+    if stored is not None:
+        return stored
+    return lead.sigma_fn(kpoint)(E, eta)
+
+
+def fiori_free_charge(blocks, sigL, sigR, E, pre_factor, fL, fR, E_ref, eta_device,
+                      potential_at_atom, device_atom_norbs):
+    """Per-atom free charge [natom] (float64) of ``density.py:268-360`` over
+    the integration points ``E`` [B] (relative to ``E_ref``) with weights
+    ``pre_factor`` [B] and lead occupations ``fL``, ``fR`` [B] at E + E_ref.
+
+    Per point and orbital o: a_D = diag i(G - G^dagger) = -2 Im G_oo,
+    a_R = diag(G_{q,K-1} Gamma_R G_{q,K-1}^dagger) (G_qq for one block),
+    G^n_oo = (a_D - a_R) f_L + a_R f_R, G^p_oo = a_D - G^n_oo; atom j gains
+    -pre/pi sum_o G^n_oo where E >= -potential_at_atom[j] (electrons) and
+    +pre/pi sum_o G^p_oo otherwise (holes).  Orbitals are counted in device
+    order, so an atom straddling block edges sums over its blocks as the
+    reference's ``get_subblock_index`` does."""
+    hd, hu, hl, sd, su, sl = blocks
+    dev = hd[0].device
+    sizes = block_sizes(hd)
+    K = len(sizes)
+    nK = sizes[-1]
+    arena = pack_arena(hd, hu, hl, sd, su, sl, dev)
+    ortho = is_orthogonal(sd, su, sl)
+    E = torch.as_tensor(E).to(device=dev, dtype=torch.float64).reshape(-1)
+    B = E.shape[0]
+    sL, nL = corner_self_energy(sigL, sizes[0], B, False, dev)
+    sR, nR = corner_self_energy(sigR, nK, B, True, dev)
+    gamR = torch.zeros(B, nK, nK, dtype=C128, device=dev)
+    gamR[:, nK - nR:, nK - nR:] = 1j * (sR - sR.mH)
+    z = (E.to(C128) + E_ref + 1j * eta_device).contiguous()
+    aD, aR = [], []
+    chunk = full_points_per_call(sizes, True, B, dev)
+    for c0 in range(0, B, chunk):
+        c = slice(c0, min(B, c0 + chunk))
+        Bc = z[c].shape[0]
+        raw = _run_full(z[c].contiguous(), arena, sizes, ortho, sL[c].contiguous(), nL,
+                        sR[c].contiguous(), nR, True)
+        grd = split_blocks(raw["grd"], Bc, [(n, n) for n in sizes])
+        lc = split_blocks(raw["lc"], Bc, [(n, nK) for n in sizes])
+        gR = gamR[c].contiguous()
+        dq, rq = [], []
+        for q in range(K):
+            dq.append(-2.0 * torch.diagonal(grd[q], dim1=-2, dim2=-1).imag)
+            X = mm(lc[q], gR)
+            rq.append(diag_of_product([(X, lc[q].mH.contiguous())], Bc, dev).real)
+        aD.append(torch.cat(dq, dim=-1))
+        aR.append(torch.cat(rq, dim=-1))
+        del raw
+    aD, aR = torch.cat(aD), torch.cat(aR)
+    fL = torch.as_tensor(fL).to(device=dev, dtype=torch.float64).reshape(B, 1)
+    fR = torch.as_tensor(fR).to(device=dev, dtype=torch.float64).reshape(B, 1)
+    gn = (aD - aR) * fL + aR * fR
+    gp = aD - gn
+    norbs = [int(x) for x in device_atom_norbs]
+    atom = torch.repeat_interleave(torch.arange(len(norbs), device=dev),
+                                   torch.as_tensor(norbs, device=dev))
+    onehot = torch.nn.functional.one_hot(atom, len(norbs)).to(torch.float64)
+    Qn, Qp = gn @ onehot, gp @ onehot
+    Ei = -torch.as_tensor(potential_at_atom).to(device=dev, dtype=torch.float64).reshape(1, -1)
+    electron = E.reshape(B, 1) >= Ei
+    pre = torch.as_tensor(pre_factor).to(device=dev, dtype=torch.float64).reshape(B, 1)
+    return (torch.where(electron, -Qn, Qp) * pre).sum(0) / math.pi
+
+
+class Fiori(object):
+    """``dpnegf/negf/density.py:234-393`` on the GPU: the free charge of the
+    Poisson-NEGF loop from a real-axis integral, every integration energy in
+    one batch (one decimation launch per lead, full sweeps with the last block
+    column, per-orbital spectral diagonals) instead of the reference's Python
+    loop with one RGF per energy."""
+
+    def __init__(self, n_gauss=None, integrate_way="direct", e_grid=None):
+        self.n_gauss = n_gauss
+        self.e_grid = e_grid
+        self.integrate_way = integrate_way
+        self.xs = None
+        self.wlg = None
+        self.e_grid_Fiori = None
+        if integrate_way == "gauss":
+            assert self.n_gauss is not None, \
+                "n_gauss must be set in the Fiori class with gauss integration"
+            self.xs, self.wlg = gauss_xw(torch.scalar_tensor(float(e_grid[0]), dtype=torch.float64),
+                                         torch.scalar_tensor(float(e_grid[-1]),
+                                                             dtype=torch.float64),
+                                         n=self.n_gauss)
+            self.e_grid_Fiori = e_grid
+            integrate_range, pre_factor = self.xs, self.wlg
+        elif integrate_way == "direct":
+            e_grid = torch.as_tensor(e_grid, dtype=torch.float64)
+            dE = e_grid[1] - e_grid[0]
+            integrate_range = e_grid
+            pre_factor = dE * torch.ones(len(e_grid), dtype=torch.float64)
+        else:
+            raise ValueError("integrate_way only supports 'gauss' and 'direct' in this version")
+        self.integrate_range = integrate_range
+        self.pre_factor = pre_factor
+
+    def density_integrate_Fiori(self, e_grid, kpoint, Vbias, block_tridiagonal, subblocks,
+                                integrate_way, deviceprop, device_atom_norbs, potential_at_atom,
+                                with_Dirichlet_leads, free_charge, E_ref, eta_lead=1e-5,
+                                eta_device=1e-5, self_energy_save_path=None):
+        """Adds each atom's free charge to ``free_charge[str(kpoint)]`` as the
+        reference does (``density.py:268-360``).  Self-energies stored by
+        ``compute_all_self_energy`` are used when present (the reference loads
+        them for Dirichlet leads), otherwise computed in one batch."""
+        E = torch.as_tensor(self.integrate_range, dtype=torch.float64)
+        dev = deviceprop.rgf_device
+        V = Vbias if Vbias is not None else torch.tensor(0.)
+        hs = deviceprop.hamiltonian.get_hs_device(kpoint, V, block_tridiagonal)
+        hd, sd, hl, su, sl, hu = [[b.to(dev) for b in blk] for blk in hs]
+        lL, lR = deviceprop.lead_L, deviceprop.lead_R
+        sigL = _lead_sigmas(lL, kpoint, E, eta_lead)
+        sigR = _lead_sigmas(lR, kpoint, E, eta_lead)
+        Ed = E.to(dev)
+        charge = fiori_free_charge((hd, hu, hl, sd, su, sl), sigL, sigR, Ed, self.pre_factor,
+                                   lL.fermi_dirac(Ed + E_ref), lR.fermi_dirac(Ed + E_ref), E_ref,
+                                   eta_device, potential_at_atom, device_atom_norbs)
+        key = str(kpoint)
+        cur = free_charge[key]
+        free_charge[key] = cur + charge.to(device=cur.device, dtype=cur.dtype) \
+            if torch.is_tensor(cur) else cur + charge.cpu().numpy()
+        return free_charge
+
+    def get_subblock_index(self, subblocks, atom_index, device_atom_norbs):
+        """Blocks an atom's orbitals occupy, with its start in the first block
+        and its end in the last (``density.py:362-393``)."""
+        cum = [0] + list(np.cumsum(subblocks))
+        pre = sum(device_atom_norbs[:atom_index])
+        last = pre + device_atom_norbs[atom_index]
+        block_index, orb_start, orb_end = [], None, None
+        for i in range(len(cum) - 1):
+            if cum[i] <= pre and last <= cum[i + 1]:
+                return [i], pre - cum[i], last - cum[i]
+            if cum[i] <= pre < cum[i + 1] < last:
+                block_index.append(i)
+                orb_start = pre - cum[i]
+                for j in range(i + 1, len(cum) - 1):
+                    block_index.append(j)
+                    if last <= cum[j + 1]:
+                        orb_end = last - cum[j]
+                        break
+        return block_index, orb_start, orb_end
