@@ -31,6 +31,7 @@ struct BGemmArgs {
   cx<double> alpha, beta;
   cx<double>* C;
   const cx<double>* Cin;  // source of the beta term, laid out as C (often C itself)
+  const int32_t* skip;    // [B] or null: batch entries with skip[b] != 0 are left untouched
   int64_t cstride;
   int ldc;
   const cx<double>* z;

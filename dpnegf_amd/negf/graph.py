@@ -9,8 +9,12 @@ lead over all points) and the fused RGF transmission sweep -- and replays them
 with no host work.  ``update`` rewrites the packed blocks in place between
 replays.  Replays and updates run on the caller's current stream.
 
-Limits: blocks <= 64 and leads <= 48 (the single-launch kernels); larger
-blocks iterate with host synchronisation and cannot be captured.
+Any block and lead size the library takes can be captured.  Blocks above 64
+use the block-operation sweep (one call per k-point, the points of each
+k-point gathered and scattered by precomputed index lists); leads above 48
+orbitals use the batched decimation, which under capture launches its full
+101 iterations with converged points frozen instead of asking the host when
+to stop (at most 65535 points per graph for either).
 """
 import torch
 
@@ -31,15 +35,17 @@ class TransmissionGraph:
         point (required with more than one system)."""
         self.bs = BTDSystems(systems)
         dev = self.bs.device
-        if max(self.bs.sizes) > nat.TRANSMISSION_NMAX:
-            raise NotImplementedError("graph capture needs block sizes <= %d"
-                                      % nat.TRANSMISSION_NMAX)
+        self.blocked = max(self.bs.sizes) > nat.TRANSMISSION_NMAX
         if len(systems) != len(leads_L) or len(systems) != len(leads_R):
             raise ValueError("one left and one right lead per system")
         if len(systems) > 1 and sys_idx is None:
             raise ValueError("sys_idx is required with more than one system")
         e = torch.as_tensor(energies).to(device=dev, dtype=C128).reshape(-1) + E_ref
         self.P = e.shape[0]
+        if self.P > 65535 and (self.blocked or max(
+                side[0][0].shape[-1] for side in (leads_L, leads_R)) > nat.LEAD_NMAX):
+            raise ValueError("at most 65535 points per graph with blocks above 64 or leads "
+                             "above 48 orbitals")
         self.e = e.contiguous()
         self.z = (self.e + 1j * float(eta_device)).contiguous()
         self.sys_idx = (None if sys_idx is None
@@ -54,8 +60,6 @@ class TransmissionGraph:
             a = side[0][2].shape[0]
             if a > min(self.bs.sizes[0], self.bs.sizes[-1]):
                 raise ValueError("coupling rows exceed the end blocks")
-            if max(n, a) > nat.LEAD_NMAX:
-                raise NotImplementedError("graph capture needs leads <= %d" % nat.LEAD_NMAX)
             ortho = all(_orthogonal(l[3].to(dev), l[4].to(dev)) for l in side)
             ws = torch.empty(nat.load().dpnegf_lead_self_energy_workspace_elems(n, a, self.P),
                              dtype=C128, device=dev)
@@ -69,12 +73,55 @@ class TransmissionGraph:
                 resid=torch.zeros(self.P, dtype=torch.float64, device=dev)))
         self.T = torch.empty(self.P, dtype=torch.float64, device=dev)
         self.status = torch.zeros(self.P, dtype=torch.int32, device=dev)
+        if self.blocked:
+            self._prepare_blocked()
         self._launch()  # eager run: validates arguments before capture
         torch.cuda.synchronize(dev)
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):  # torch's capture stream is current inside
             self._launch()
         torch.cuda.synchronize(dev)
+
+    def _prepare_blocked(self):
+        """Per k-point: its point indices and the gathered z, Sigma_L, Sigma_R,
+        T, status and work buffers of the block-operation sweep."""
+        dev = self.bs.device
+        K = len(self.bs.sizes)
+        idx = (self.sys_idx if self.sys_idx is not None
+               else torch.zeros(self.P, dtype=torch.int32, device=dev)).cpu()
+        nL = self.leads[0]["a"]
+        nR = self.leads[1]["a"]
+        self.parts = []
+        for s in range(self.bs.nsys):
+            sel = (idx == s).nonzero().reshape(-1)
+            B = int(sel.numel())
+            if B == 0:
+                continue
+            nwork = nat.load().dpnegf_rgf_transmission_blocked_workspace_elems(
+                K, self.bs._h_bsz, B)
+            self.parts.append(dict(
+                s=s, sel=sel.to(dev), B=B,
+                z=torch.empty(B, dtype=C128, device=dev),
+                sL=torch.empty(B, nL, nL, dtype=C128, device=dev),
+                sR=torch.empty(B, nR, nR, dtype=C128, device=dev),
+                T=torch.empty(B, dtype=torch.float64, device=dev),
+                st=torch.zeros(B, dtype=torch.int32, device=dev),
+                work=torch.empty(nwork, dtype=C128, device=dev)))
+
+    def _launch_blocked(self, st):
+        K = len(self.bs.sizes)
+        sL, sR = self.leads[0]["sigma"], self.leads[1]["sigma"]
+        for p in self.parts:
+            torch.index_select(self.z, 0, p["sel"], out=p["z"])
+            torch.index_select(sL, 0, p["sel"], out=p["sL"])
+            torch.index_select(sR, 0, p["sel"], out=p["sR"])
+            arena = self.bs.arena[p["s"] * self.bs.stride:(p["s"] + 1) * self.bs.stride]
+            nat.call("dpnegf_rgf_transmission_blocked", st, K, self.bs._h_bsz, nat.ptr(arena),
+                     p["B"], nat.ptr(p["z"]), nat.ptr(p["sL"]), p["sL"].shape[-1],
+                     nat.ptr(p["sR"]), p["sR"].shape[-1], int(self.bs.ortho), nat.ptr(p["T"]),
+                     nat.ptr(p["work"]), nat.ptr(p["st"]))
+            self.T.index_copy_(0, p["sel"], p["T"])
+            self.status.index_copy_(0, p["sel"], p["st"])
 
     def _launch(self):
         st = nat.stream_ptr(self.bs.device)
@@ -84,6 +131,9 @@ class TransmissionGraph:
                      self.eta_lead, float(LS_STOP), nat.ptr(L["ws"]), nat.ptr(L["gs"]),
                      nat.ptr(L["sigma"]), L["ortho"], nat.ptr(L["iters"]), nat.ptr(L["status"]),
                      nat.ptr(L["resid"]))
+        if self.blocked:
+            self._launch_blocked(st)
+            return
         sL, sR = self.leads[0]["sigma"], self.leads[1]["sigma"]
         nat.call("dpnegf_rgf_transmission", st, nat.DPNEGF_FP64, len(self.bs.sizes),
                  self.bs._h_bsz, nat.ptr(self.bs.bsz), nat.ptr(self.bs.arena), self.bs.stride,

@@ -154,7 +154,7 @@ def test_adjoint_ribbon64_K64_bias_window():
         T = transmission(_f64(E), params[:K], params[K:2 * K - 1], params[2 * K - 1:], dv(s.sd),
                          dv(s.su), dv(s.sl), sL.to(dev), sR.to(dev), eta, checkpoint=mode)
         Jg = (T * w.to(dev)).sum()
-        assert abs(float(Jg.detach()) - float(J.detach())) < 1e-10 * max(1.0, abs(float(J)))
+        assert abs(float(Jg.detach()) - float(J.detach())) < 1e-10 * max(1.0, abs(float(J.detach())))
         grads[mode] = [g.cpu() for g in torch.autograd.grad(Jg, params)]
         err = max((r - g).abs().max().item() for r, g in zip(ref, grads[mode])) / scale
         print(f"  mode {mode}: J = {float(Jg):.6f}, max rel grad err vs autograd {err:.2e}",
