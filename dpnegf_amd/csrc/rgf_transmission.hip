@@ -33,20 +33,22 @@ constexpr double REFINE_TOL = 1e-4;
 // refined by one FP64 Newton step, X = X0 + X0 (I - A X0); a block whose FP32
 // residual exceeds REFINE_TOL is inverted in FP64 instead (status bit
 // STATUS_REFINE_FALLBACK).  Products and the carried P stay FP64.
-template <typename R, int NMAX, bool ORTHO, bool FIXED, bool REFINE = false>
-__global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(NMAX <= 48 && !REFINE ? 2 : 1)))
+// NTH: threads per workgroup (NTH / 64 waves share every product and update).
+template <typename R, int NMAX, bool ORTHO, bool FIXED, bool REFINE = false, int NTH = 256>
+__global__ __launch_bounds__(NTH)
+__attribute__((amdgpu_waves_per_eu(NMAX <= 48 && !REFINE ? 2 * NTH / 256 : NTH / 256)))
 void rgf_transmission_kernel(RgfArgs a) {
   static_assert(!REFINE || (sizeof(R) == 8 && NMAX <= 48), "refinement: FP64 kernel, NMAX <= 48");
+  constexpr int NW = NTH / WAVE;
   constexpr int LD = NMAX + 1;
   constexpr int NT = (NMAX / 16) * (NMAX / 16);
-  constexpr int MAXT_STACK = (2 * NT + 3) / 4;
-  constexpr int MAXT_SQ = (NT + 3) / 4;
-  constexpr int MAXR = gj_maxr(NMAX, 256);
+  constexpr int MAXT_STACK = (2 * NT + NW - 1) / NW;
+  constexpr int MAXT_SQ = (NT + NW - 1) / NW;
+  constexpr int MAXR = gj_maxr(NMAX, NTH);
   __shared__ cx<R> PG[2 * NMAX * LD];
   __shared__ GJScratch<R, NMAX> gjs;
   __shared__ GJBScratch<R, NMAX> gjb;
-  __shared__ double red[8];
+  __shared__ double red[NW > 8 ? NW : 8];
   // FP32 copy / inverse of the block and the FP64 residual (refined variant only)
   constexpr int NR = REFINE ? NMAX * LD : 1;
   __shared__ cx<float> XF[NR];
@@ -99,10 +101,10 @@ void rgf_transmission_kernel(RgfArgs a) {
   // FIXED: blocked inverse of g in place; otherwise the register-resident one
   auto inverse_full = [&](int n) __attribute__((always_inline)) -> int {
     if constexpr (FIXED)
-      return wg_gj_inverse_blocked<R, 256, NMAX, LD>(
+      return wg_gj_inverse_blocked<R, NTH, NMAX, LD>(
           g, [&](int i, int k, cx<R> v) { g[i * LD + k] = v; }, &gjb);
     else
-      return wg_gj_inverse<R, 256, NMAX, MAXR>(
+      return wg_gj_inverse<R, NTH, NMAX, MAXR>(
           n, [&](int i, int k) { return g[i * LD + k]; },
           [&](int i, int k, cx<R> v) { g[i * LD + k] = v; }, &gjs);
   };
@@ -111,17 +113,17 @@ void rgf_transmission_kernel(RgfArgs a) {
     if constexpr (!REFINE) {
       return inverse_full(n);
     } else {
-      for (int e = threadIdx.x; e < n * n; e += 256) {
+      for (int e = threadIdx.x; e < n * n; e += NTH) {
         const int i = e / n, k = e % n;
         XF[i * LD + k] = cvt<float>(g[i * LD + k]);
       }
       __syncthreads();
       int sg;
       if constexpr (FIXED)
-        sg = wg_gj_inverse_blocked<float, 256, NMAX, LD>(
+        sg = wg_gj_inverse_blocked<float, NTH, NMAX, LD>(
             XF, [&](int i, int k, cx<float> v) { XF[i * LD + k] = v; }, &gjbf);
       else
-        sg = wg_gj_inverse<float, 256, NMAX, gj_maxr(NMAX, 256)>(
+        sg = wg_gj_inverse<float, NTH, NMAX, gj_maxr(NMAX, NTH)>(
             n, [&](int i, int k) { return XF[i * LD + k]; },
             [&](int i, int k, cx<float> v) { XF[i * LD + k] = v; }, &gjsf);
       __syncthreads();
@@ -134,8 +136,8 @@ void rgf_transmission_kernel(RgfArgs a) {
         RS[i * LD + j] = r;
         rmax = fmax(rmax, abs1(r));
       };
-      if constexpr (FIXED) wg_gemm_fixed<double, 4, NMAX, NMAX, NMAX, false, false>(ra, rb, re);
-      else wg_gemm<double, 4, MAXT_SQ>(n, n, n, ra, rb, re);
+      if constexpr (FIXED) wg_gemm_fixed<double, NW, NMAX, NMAX, NMAX, false, false>(ra, rb, re);
+      else wg_gemm<double, NW, MAXT_SQ>(n, n, n, ra, rb, re);
       rmax = block_max(rmax, red);  // ends with the residual visible to all
       if (sg || !(rmax < REFINE_TOL)) {  // uniform: singular, NaN or too far for one step
         refine_fallback = 1;
@@ -145,20 +147,20 @@ void rgf_transmission_kernel(RgfArgs a) {
       auto xa = [&](int i, int k) { return cvt<double>(XF[i * LD + k]); };
       auto xb = [&](int k, int j) { return RS[k * LD + j]; };
       auto xe = [&](int i, int j, cx<double> v) { g[i * LD + j] = cvt<double>(XF[i * LD + j]) + v; };
-      if constexpr (FIXED) wg_gemm_fixed<double, 4, NMAX, NMAX, NMAX, false, false>(xa, xb, xe);
-      else wg_gemm<double, 4, MAXT_SQ>(n, n, n, xa, xb, xe);
+      if constexpr (FIXED) wg_gemm_fixed<double, NW, NMAX, NMAX, NMAX, false, false>(xa, xb, xe);
+      else wg_gemm<double, NW, MAXT_SQ>(n, n, n, xa, xb, xe);
       __syncthreads();
       return 0;
     }
   };
-  for (int e = threadIdx.x; e < n0 * n0; e += 256) {
+  for (int e = threadIdx.x; e < n0 * n0; e += NTH) {
     const int i = e / n0, k = e % n0;
     g[i * LD + k] = cvt<R>(-Mval(hd0, sd0, n0, i, k, true, K == 1));
   }
   __syncthreads();
   int sing = inverse_g(n0);
   __syncthreads();
-  for (int e = threadIdx.x; e < n0 * n0; e += 256) {
+  for (int e = threadIdx.x; e < n0 * n0; e += NTH) {
     const int i = e / n0, k = e % n0;
     P[i * LD + k] = g[i * LD + k];
   }
@@ -184,8 +186,8 @@ void rgf_transmission_kernel(RgfArgs a) {
       return cvt<R>(u);
     };
     auto e1 = [&](int i, int j, cx<R> v) { PG[i * LD + j] = v; };
-    if constexpr (FIXED) wg_gemm_fixed<R, 4, 2 * NMAX, NMAX, NMAX, false, true>(a1, b1, e1);
-    else wg_gemm<R, 4, MAXT_STACK>(n0 + nq, n1, nq, a1, b1, e1);
+    if constexpr (FIXED) wg_gemm_fixed<R, NW, 2 * NMAX, NMAX, NMAX, false, true>(a1, b1, e1);
+    else wg_gemm<R, NW, MAXT_STACK>(n0 + nq, n1, nq, a1, b1, e1);
     __syncthreads();
     // g <- -M_{q+1} - (L_q - z SL_q) Y
     auto a2 = [&](int i, int k) {
@@ -197,16 +199,16 @@ void rgf_transmission_kernel(RgfArgs a) {
     auto e2 = [&](int i, int j, cx<R> v) {
       g[i * LD + j] = cvt<R>(-(cvt<double>(v) + Mval(hd1, sd1, n1, i, j, false, last)));
     };
-    if constexpr (FIXED) wg_gemm_fixed<R, 4, NMAX, NMAX, NMAX, true, false>(a2, b2, e2);
-    else wg_gemm<R, 4, MAXT_SQ>(n1, n1, nq, a2, b2, e2);
+    if constexpr (FIXED) wg_gemm_fixed<R, NW, NMAX, NMAX, NMAX, true, false>(a2, b2, e2);
+    else wg_gemm<R, NW, MAXT_SQ>(n1, n1, nq, a2, b2, e2);
     __syncthreads();
     sing |= inverse_g(n1);  // ends with a workgroup barrier
     // P <- P g_{q+1}
     auto a3 = [&](int i, int k) { return P[i * LD + k]; };
     auto b3 = [&](int k, int j) { return g[k * LD + j]; };
     auto e3 = [&](int i, int j, cx<R> v) { P[i * LD + j] = v; };
-    if constexpr (FIXED) wg_gemm_fixed<R, 4, NMAX, NMAX, NMAX, false, false>(a3, b3, e3);
-    else wg_gemm<R, 4, MAXT_SQ>(n0, n1, n1, a3, b3, e3);
+    if constexpr (FIXED) wg_gemm_fixed<R, NW, NMAX, NMAX, NMAX, false, false>(a3, b3, e3);
+    else wg_gemm<R, NW, MAXT_SQ>(n0, n1, n1, a3, b3, e3);
     __syncthreads();
     off = off1;
   }
@@ -214,13 +216,13 @@ void rgf_transmission_kernel(RgfArgs a) {
   // T = Re sum_{a,b} GammaL[a][b] (Q GammaR Q^dagger)[b][a],  Q = P[0:nL, c0:c0+nR]
   const int c0 = nlast - nR;
   cx<R>* V = g;
-  wg_gemm<R, 4, MAXT_SQ>(
+  wg_gemm<R, NW, MAXT_SQ>(
       nL, nR, nR, [&](int i, int k) { return P[i * LD + c0 + k]; },
       [&](int k, int j) { return cvt<R>(gamma_of(sR, nR, k, j)); },
       [&](int i, int j, cx<R> v) { V[i * LD + j] = v; });
   __syncthreads();
   double acc = 0.0;
-  wg_gemm<R, 4, MAXT_SQ>(
+  wg_gemm<R, NW, MAXT_SQ>(
       nL, nL, nR, [&](int i, int k) { return V[i * LD + k]; },
       [&](int k, int j) { return conj(P[j * LD + c0 + k]); },
       [&](int i, int j, cx<R> x) {
@@ -235,16 +237,20 @@ void rgf_transmission_kernel(RgfArgs a) {
   }
 }
 
-template <typename R, int NMAX, bool FIXED, bool REFINE = false>
+template <typename R, int NMAX, bool FIXED, bool REFINE = false, int NTH = 256>
 static hipError_t launch_nmax(const RgfArgs& a, bool ortho, hipStream_t st) {
-  dim3 grid(a.P), block(256);
+  dim3 grid(a.P), block(NTH);
   if (ortho)
-    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, true, FIXED, REFINE>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, true, FIXED, REFINE, NTH>), grid, block, 0,
+                       st, a);
   else
-    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, false, FIXED, REFINE>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((rgf_transmission_kernel<R, NMAX, false, FIXED, REFINE, NTH>), grid, block, 0,
+                       st, a);
   return hipGetLastError();
 }
 
+// 256 threads: at NMAX = 48, 512 threads (4 waves per SIMD, 128 registers with
+// spills) measured 87.5 ms against 76.5 ms on the ribbon W = 48, K = 128 sweep.
 template <typename R, int NMAX>
 static hipError_t launch_sel(const RgfArgs& a, bool uniform, int nmax, bool ortho, hipStream_t st) {
   if (uniform && nmax == NMAX) return launch_nmax<R, NMAX, true>(a, ortho, st);
