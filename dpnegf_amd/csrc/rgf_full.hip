@@ -240,6 +240,32 @@ __global__ __launch_bounds__(NT, MIN_WAVES) void binv_kernel(cx<double>* A, int 
   if (threadIdx.x == 0 && status && s) atomicOr(&status[blockIdx.x], STATUS_SINGULAR);
 }
 
+// Inverse of one n x n block per workgroup, 64 < n <= NP: the block goes to LDS
+// padded to NP with an identity block (the padded rows and columns never
+// pivot the block's columns, so the n x n corner of the inverse is A^-1), is
+// inverted there by the blocked Gauss-Jordan with matrix-core rank-4
+// updates, and its n x n corner is written back in natural order.
+template <int NP, int NT>
+__global__ __launch_bounds__(NT) void binv_lds_kernel(cx<double>* A, int n, int64_t stride,
+                                                      int32_t* status, const int32_t* skip) {
+  constexpr int LD = NP + 1;
+  if (skip && skip[blockIdx.x]) return;  // uniform per workgroup, before any barrier
+  __shared__ cx<double> M[NP * LD];
+  __shared__ GJBScratch<double, NP> gjb;
+  cx<double>* G = A + (int64_t)blockIdx.x * stride;
+  for (int e = threadIdx.x; e < NP * NP; e += NT) {
+    const int i = e / NP, k = e % NP;
+    M[i * LD + k] = (i < n && k < n) ? G[(int64_t)i * n + k]
+                                     : cx<double>{i == k ? 1.0 : 0.0, 0.0};
+  }
+  __syncthreads();
+  const int s = wg_gj_inverse_blocked<double, NT, NP, LD>(
+      M, [&](int i, int k, cx<double> v) {
+        if (i < n && k < n) G[(int64_t)i * n + k] = v;
+      }, &gjb);
+  if (threadIdx.x == 0 && status && s) atomicOr(&status[blockIdx.x], STATUS_SINGULAR);
+}
+
 // D[b] = -(H - z_b S) - Pb[b] - corner(SigmaL) - corner(SigmaR), Pb ([B, n, n],
 // the energy-dependent probe self-energy seP) optional
 __global__ void bbuild_kernel(cx<double>* D, const cx<double>* H, const cx<double>* S, int n,
@@ -689,11 +715,11 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
   else if (n <= 32) hipLaunchKernelGGL((binv_kernel<32, 256>), dim3(B), dim3(256), 0, st, A, n, s, status, skip);
   else if (n <= 64)  // 8 rows per thread and 71 registers: three workgroups per CU
     hipLaunchKernelGGL((binv_kernel<64, 512, 4>), dim3(B), dim3(512), 0, st, A, n, s, status, skip);
-  // The sweep's n pivot steps (two workgroup barriers each) are latency-bound:
-  // up to 96, 512 threads with registers capped at 128 let two workgroups
-  // share a CU and fill each other's gaps.
+  // 65..96: blocked Gauss-Jordan in LDS with matrix-core rank-4 updates (the
+  // scalar register-resident sweep above is latency-bound at n pivot steps
+  // with two workgroup barriers each)
   else if (n <= 96)
-    hipLaunchKernelGGL((binv_kernel<96, 512, 4>), dim3(B), dim3(512), 0, st, A, n, s, status, skip);
+    hipLaunchKernelGGL((binv_lds_kernel<96, 512>), dim3(B), dim3(512), 0, st, A, n, s, status, skip);
   else if (n <= GJ_NMAX_LARGE)
     hipLaunchKernelGGL((binv_kernel<GJ_NMAX_LARGE, 1024>), dim3(B), dim3(1024), 0, st, A, n, s, status, skip);
   else if (n <= BLOCK_NMAX) {
@@ -1130,7 +1156,20 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
   hipLaunchKernelGGL(ls_zs_minus_kernel, dim3(gridn), dim3(256), 0, st, W, epss, a.lead,
                      a.lead_stride, sys_idx, a.e, n, B, a.eta, nullptr);
   CHK(hipGetLastError());
-  CHK(inverse(st, W, n, B, status, iws));
+  if (n <= GJ_NMAX_LARGE) {
+    // one FP64 Newton step on the one-workgroup inverse, X <- X + X (I - A X)
+    // (above GJ_NMAX_LARGE the two-level inverse takes its own)
+    CHK(hipMemcpyAsync(X1, W, sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
+    CHK(inverse(st, W, n, B, status, iws));
+    CHK(gemm(st, B, n, n, n, M(X1), M(W), mone, zero, X2, nn, n, nullptr));
+    hipLaunchKernelGGL(badd_identity_kernel, dim3(grid_of((int64_t)B * n)), dim3(256), 0, st, X2, n,
+                       B, nullptr);
+    CHK(hipGetLastError());
+    CHK(hipMemcpyAsync(Tm, W, sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
+    CHK(gemm(st, B, n, n, n, M(Tm), M(X2), one, one, W, nn, n, nullptr));
+  } else {
+    CHK(inverse(st, W, n, B, status, iws));
+  }
   CHK(hipMemcpyAsync(a.gs, W, sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
   const cx<double>* gs = a.gs;
   // residual: X1 = (z s01 - h01) g_s ; X2 = X1 (z s10 - h10) ; R = (zS - H - X2) g_s

@@ -474,14 +474,154 @@ __device__ __forceinline__ int wg_gj_blocked_core(cx<R>* A, GJBScratch<R, N>* sh
   return sh->singular;
 }
 
+// The same blocked Gauss-Jordan for 64 < N <= 128: the sweeping wave holds two
+// rows per lane (rows lane and lane + 64), and the pivot key carries the row in
+// its low 7 bits (16 mantissa bits of |re| + |im| compared, lowest row on
+// ties).  Steps, multipliers, pivot-row scaling, forward substitution and the
+// rank-4 matrix-core update are those of wg_gj_blocked_core.
+template <typename R, int NT, int N, int LDA>
+__device__ __forceinline__ int wg_gj_blocked_core_tall(cx<R>* A, GJBScratch<R, N>* sh) {
+  constexpr int BW = 4;
+  constexpr int RPL = 2;
+  constexpr int NW = NT / WAVE;
+  constexpr int NTILE = ((N + 15) / 16) * ((N + 15) / 16);
+  constexpr int MAXT = (NTILE + NW - 1) / NW;
+  static_assert(N > WAVE && N <= RPL * WAVE && N % BW == 0,
+                "tall blocked Gauss-Jordan: 64 < N <= 128, multiple of 4");
+  const int t = threadIdx.x;
+  const int lane = t & (WAVE - 1);
+  const int wave = t / WAVE;
+  if (t == 0) sh->singular = 0;
+  bool used[RPL] = {false, false};
+  bool row_ok[RPL];
+#pragma unroll
+  for (int j = 0; j < RPL; ++j) row_ok[j] = lane + WAVE * j < N;
+  for (int j0 = 0; j0 < N; j0 += BW) {
+    if (wave == 0) {
+      cx<R> v[RPL][BW], F[RPL][BW], dinv[BW], L[BW][BW];
+      int pr[BW];
+      int pstep[RPL] = {BW, BW};  // step at which each of this lane's rows pivots (BW: none)
+#pragma unroll
+      for (int j = 0; j < RPL; ++j)
+#pragma unroll
+        for (int c = 0; c < BW; ++c)
+          v[j][c] = row_ok[j] ? A[(lane + WAVE * j) * LDA + j0 + c] : cx<R>{0, 0};
+#pragma unroll
+      for (int jj = 0; jj < BW; ++jj) {
+        unsigned key = 0;
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+          if (row_ok[j] && !used[j]) {
+            const float kf = (float)pivot_key(v[j][jj]);
+            const unsigned kj = (((__float_as_uint(kf) >> 7) + 1u) << 7) |
+                                (unsigned)(2 * WAVE - 1 - (lane + WAVE * j));
+            key = kj > key ? kj : key;
+          }
+        }
+        key = wave_max_u32(key);
+        int p = 2 * WAVE - 1 - (int)(key & 127u);
+        p = p < N ? p : N - 1;
+        const int pl = p & (WAVE - 1);
+        const bool hi = p >= WAVE;  // uniform: the pivot row is the lane's second row
+        // broadcast both candidates and select the value (selecting the
+        // register by index would put the arrays in scratch memory)
+        auto pick = [&](cx<R> x0, cx<R> x1) {
+          const cx<R> a = bcast_lane(x0, pl), b = bcast_lane(x1, pl);
+          return hi ? b : a;
+        };
+        pr[jj] = p;
+        bool isp[RPL];
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+          isp[j] = (lane + WAVE * j == p);
+          if (isp[j]) { used[j] = true; pstep[j] = jj; }
+          F[j][jj] = v[j][jj];
+        }
+#pragma unroll
+        for (int k2 = 0; k2 < jj; ++k2) L[jj][k2] = pick(F[0][k2], F[1][k2]);
+        const cx<R> d = pick(F[0][jj], F[1][jj]);
+        dinv[jj] = recip_nr(d);
+        if (lane == 0) {
+          sh->piv[j0 + jj] = p;
+          sh->pr[jj] = p;
+          if (d.re == R(0) && d.im == R(0)) sh->singular = 1;
+        }
+        cx<R> m[RPL];
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+          const cx<R> m0 = F[j][jj] * dinv[jj];
+          m[j] = cx<R>{isp[j] ? R(0) : m0.re, isp[j] ? R(0) : m0.im};
+        }
+#pragma unroll
+        for (int c = 0; c < BW; ++c) {
+          if (c == jj) continue;
+          const cx<R> pc = pick(v[0][c], v[1][c]);
+#pragma unroll
+          for (int j = 0; j < RPL; ++j) {
+            v[j][c].re = fma(-m[j].re, pc.re, fma(m[j].im, pc.im, v[j][c].re));
+            v[j][c].im = fma(-m[j].re, pc.im, fma(-m[j].im, pc.re, v[j][c].im));
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < RPL; ++j)
+          v[j][jj] = cx<R>{isp[j] ? R(1) : -m[j].re, isp[j] ? R(0) : -m[j].im};
+      }
+#pragma unroll
+      for (int j = 0; j < RPL; ++j) {
+        cx<R> sc = {R(1), R(0)};
+#pragma unroll
+        for (int k = 0; k < BW; ++k)
+          if (pstep[j] == k) sc = dinv[k];
+        if (row_ok[j]) {
+          const int row = lane + WAVE * j;
+#pragma unroll
+          for (int c = 0; c < BW; ++c) {
+            A[row * LDA + j0 + c] = v[j][c] * sc;
+            sh->F[row][c] = (pstep[j] < BW && c <= pstep[j]) ? cx<R>{0, 0} : F[j][c] * sc;
+          }
+        }
+      }
+      // pivot rows of the other columns: forward substitution in step order
+      for (int c = lane; c < N; c += WAVE) {
+        if (c >= j0 && c < j0 + BW) continue;
+        cx<R> u[BW];
+#pragma unroll
+        for (int k = 0; k < BW; ++k) {
+          cx<R> acc = A[pr[k] * LDA + c];
+#pragma unroll
+          for (int k2 = 0; k2 < k; ++k2) acc = acc - L[k][k2] * u[k2];
+          u[k] = acc * dinv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < BW; ++k) A[pr[k] * LDA + c] = u[k];
+      }
+    }
+    __syncthreads();
+    const int p0 = sh->pr[0], p1 = sh->pr[1], p2 = sh->pr[2], p3 = sh->pr[3];
+    wg_gemm<R, NW, MAXT, true, 1>(
+        N, N, BW, [&](int i, int k) { return sh->F[i][k]; },
+        [&](int k, int c) {
+          const int pk = k == 0 ? p0 : k == 1 ? p1 : k == 2 ? p2 : p3;
+          return A[pk * LDA + c];
+        },
+        [&](int i, int c, cx<R> val) {
+          if (c < j0 || c >= j0 + BW) A[i * LDA + c] = A[i * LDA + c] - val;
+        });
+    __syncthreads();
+  }
+  return sh->singular;
+}
+
 // In-place inverse of the N x N matrix A (LDS, leading dimension LDA) by the
-// blocked Gauss-Jordan above; store(i, k, v) receives the inverse in natural
+// blocked Gauss-Jordan above (N <= 64) or its two-rows-per-lane form (N <= 128); store(i, k, v) receives the inverse in natural
 // order (A^-1[pinv[i]][piv[k]] = Y[i][k] for the swept Y).  Returns nonzero if
 // an exactly-zero pivot was met.
 template <typename R, int NT, int N, int LDA, class FS>
 __device__ __forceinline__ int wg_gj_inverse_blocked(cx<R>* A, FS store, GJBScratch<R, N>* sh) {
   const int t = threadIdx.x;
-  const int sing = wg_gj_blocked_core<R, NT, N, LDA>(A, sh);
+  int sing;
+  if constexpr (N <= WAVE) sing = wg_gj_blocked_core<R, NT, N, LDA>(A, sh);
+  else sing = wg_gj_blocked_core_tall<R, NT, N, LDA>(A, sh);
   for (int j = t; j < N; j += NT) sh->pinv[j] = j;
   __syncthreads();
   for (int j = t; j < N; j += NT) {
