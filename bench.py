@@ -197,6 +197,19 @@ def bench_multiorbital(args):
         if ev is not None:
             ev[3].record()
 
+    tuned = None
+    if args.autotune:
+        # kernel shapes (and the sweep's batch) tuned on first use, outside the timed region
+        from dpnegf_amd.negf import autotune
+        os.environ["DPNEGF_AUTOTUNE"] = "1"
+        t0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        picks = {}
+        for res_ in autotune._kernel_cache.values():
+            picks.update(res_)
+        tuned = {"tuning_s": round(time.perf_counter() - t0, 2), "keys": len(picks),
+                 "non_default": sorted([list(k) + [c] for k, (c, _) in picks.items() if c])}
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -237,6 +250,8 @@ def bench_multiorbital(args):
         "T_checksum": float(T.sum().item()),
         "status_nonzero": int((status != 0).sum().item()),
     }
+    if tuned is not None:
+        out["autotune"] = tuned
     print(json.dumps(out))
 
 
@@ -467,6 +482,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="multiorbital: tune kernel shapes and the sweep's batch before timing")
     ap.add_argument("--norb", type=int, default=3, help="multiorbital: orbitals per site")
     ap.add_argument("--widths", default="32,40,48,56", help="multiorbital: layer widths (sites)")
     ap.add_argument("--layers", type=int, default=40, help="multiorbital: number of layers")

@@ -10,7 +10,7 @@ import torch
 
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libdpnegf_amd.so")
 
-ABI_VERSION = 3  # DPNEGF_ABI_VERSION of include/dpnegf_amd.h this binding matches
+ABI_VERSION = 4  # DPNEGF_ABI_VERSION of include/dpnegf_amd.h this binding matches
 
 DPNEGF_FP64 = 0
 DPNEGF_MIXED = 1
@@ -22,6 +22,11 @@ STATUS_LS_RESID_FAIL = 4
 STATUS_LS_RESID_WARN = 8
 STATUS_REFINE_FALLBACK = 16
 STATUS_BAD_INDEX = 32
+
+TUNE_GEMM_TILE = 0
+TUNE_INVERSE = 1
+TILE_32x32, TILE_32x64, TILE_64x32, TILE_64x64 = 1, 2, 3, 4
+INV_REGISTER, INV_LDS = 1, 2
 
 TRANSMISSION_NMAX = 64
 LEAD_NMAX = 48
@@ -63,6 +68,12 @@ _SIGS = {
     "dpnegf_zinv_batched": ([_vp, _vp, _i32, _i32, _vp], _i32),
     "dpnegf_rgf_transmission_blocked_workspace_elems": ([_i32, _vp, _i32], _i64),
     "dpnegf_autotune_query": ([_i32, _vp, _i64], _i64),
+    "dpnegf_tune_set": ([_i32, _i32, _i32, _i32, _i32], _i32),
+    "dpnegf_tune_get": ([_i32, _i32, _i32, _i32], _i32),
+    "dpnegf_tune_builtin": ([_i32, _i32, _i32, _i32], _i32),
+    "dpnegf_tune_clear": ([], None),
+    "dpnegf_tune_record": ([_i32], None),
+    "dpnegf_tune_recorded": ([_vp, _i32], _i32),
     "dpnegf_rgf_transmission_blocked": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _i32,
                                          _vp, _vp, _vp], _i32),
 }

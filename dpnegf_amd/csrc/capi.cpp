@@ -464,6 +464,28 @@ int64_t dpnegf_autotune_query(int K, const int32_t* h_bsz, int64_t mem_bytes) {
   return b > 65535 ? 65535 : b;
 }
 
+int dpnegf_tune_set(int kind, int m, int n, int k, int choice) {
+  if (negf::tune_set(kind, m, n, k, choice))
+    return fail(DPNEGF_ERR_ARG, "tune_set: bad entry kind=%d (%d, %d, %d) choice=%d", kind, m, n,
+                k, choice);
+  return 0;
+}
+
+int dpnegf_tune_get(int kind, int m, int n, int k) { return negf::tune_get(kind, m, n, k); }
+
+int dpnegf_tune_builtin(int kind, int m, int n, int k) {
+  return negf::tune_builtin(kind, m, n, k);
+}
+
+void dpnegf_tune_clear(void) { negf::tune_clear(); }
+
+void dpnegf_tune_record(int on) { negf::tune_record(on != 0); }
+
+int dpnegf_tune_recorded(int32_t* out, int cap) {
+  if (cap < 0 || (cap > 0 && !out)) return fail(DPNEGF_ERR_ARG, "bad output buffer");
+  return negf::tune_recorded(out, cap);
+}
+
 int dpnegf_rgf_transmission_blocked(void* stream, int K, const int32_t* h_bsz, const void* arena,
                                     int B, const void* z, const void* sigL, int nL,
                                     const void* sigR, int nR, int ortho, double* T, void* work,

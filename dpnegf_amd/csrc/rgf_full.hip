@@ -13,7 +13,11 @@
 // form H - z_b S), binv (Gauss-Jordan inverse, one workgroup per matrix),
 // bbuild (-(H - z_b S) - Sigma corners).
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -638,6 +642,93 @@ __global__ void ls_coupling_kernel(cx<double>* out, const cx<double>* lead, int6
 // ---------------------------------------------------------------------------------
 // host side
 
+// Tuning table.  Lookups take the mutex only while the table holds an entry or
+// recording is on; otherwise a driver pays two relaxed atomic loads.
+namespace {
+typedef std::array<int, 4> TuneKey;
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tune;
+std::vector<TuneKey> g_tune_seen;
+std::atomic<int> g_tune_entries{0};
+std::atomic<bool> g_tune_recording{false};
+
+int tune_lookup(int kind, int m, int n, int k) {
+  if (g_tune_entries.load(std::memory_order_relaxed) == 0 &&
+      !g_tune_recording.load(std::memory_order_relaxed))
+    return 0;
+  const TuneKey key{kind, m, n, k};
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (g_tune_recording.load(std::memory_order_relaxed) &&
+      std::find(g_tune_seen.begin(), g_tune_seen.end(), key) == g_tune_seen.end())
+    g_tune_seen.push_back(key);
+  const auto it = g_tune.find(key);
+  return it == g_tune.end() ? 0 : it->second;
+}
+}  // namespace
+
+// Built-in choices: 32-wide tiles in a dimension below 256 whose last 64-wide
+// tile would be at most half full (there the padding is a large share of the
+// work; from 256 on it is at most 11%, and 64-wide tiles run faster than
+// that); the LDS blocked inverse for 65..96 (the scalar register-resident
+// sweep is latency-bound at n pivot steps with two workgroup barriers each)
+static int builtin_tile(int m, int n) {
+  const bool fr1 = m < 256 && m % 64 != 0 && m % 64 <= 32;
+  const bool fc1 = n < 256 && n % 64 != 0 && n % 64 <= 32;
+  return fr1 ? (fc1 ? TILE_32x32 : TILE_32x64) : (fc1 ? TILE_64x32 : TILE_64x64);
+}
+
+static int builtin_inverse(int n) { return (n > 64 && n <= 96) ? INV_LDS : INV_REGISTER; }
+
+int tune_builtin(int kind, int m, int n, int k) {
+  if (kind == TUNE_GEMM_TILE && m >= 1 && n >= 1 && k >= 1) return builtin_tile(m, n);
+  if (kind == TUNE_INVERSE && m >= 1 && m <= GJ_NMAX_LARGE && n == 0 && k == 0)
+    return builtin_inverse(m);
+  return 0;
+}
+
+int tune_set(int kind, int m, int n, int k, int choice) {
+  if (kind == TUNE_GEMM_TILE) {
+    if (m < 1 || n < 1 || k < 1 || choice < 0 || choice > TILE_64x64) return -1;
+  } else if (kind == TUNE_INVERSE) {
+    if (n != 0 || k != 0 || m < 1 || m > GJ_NMAX_LARGE || choice < 0 || choice > INV_LDS) return -1;
+    if (choice == INV_LDS && m > 96) return -1;
+  } else {
+    return -1;
+  }
+  const TuneKey key{kind, m, n, k};
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (choice == 0) g_tune.erase(key);
+  else g_tune[key] = choice;
+  g_tune_entries.store((int)g_tune.size(), std::memory_order_relaxed);
+  return 0;
+}
+
+int tune_get(int kind, int m, int n, int k) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const auto it = g_tune.find(TuneKey{kind, m, n, k});
+  return it == g_tune.end() ? 0 : it->second;
+}
+
+void tune_clear() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
+  g_tune_entries.store(0, std::memory_order_relaxed);
+}
+
+void tune_record(bool on) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (on) g_tune_seen.clear();
+  g_tune_recording.store(on, std::memory_order_relaxed);
+}
+
+int tune_recorded(int32_t* out, int cap) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const int cnt = (int)g_tune_seen.size();
+  for (int i = 0; i < cnt && i < cap; ++i)
+    for (int j = 0; j < 4; ++j) out[4 * i + j] = g_tune_seen[i][j];
+  return cnt;
+}
+
 static hipError_t gemm(hipStream_t st, int B, int m, int n, int k, Opnd A, Opnd Bo,
                        cx<double> alpha, cx<double> beta, cx<double>* C, int64_t cstride,
                        int ldc, const cx<double>* z, const cx<double>* Cin,
@@ -648,11 +739,11 @@ static hipError_t gemm(hipStream_t st, int B, int m, int n, int k, Opnd A, Opnd 
   a.m = m; a.n = n; a.k = k; a.B = B;
   a.A = A; a.B_ = Bo; a.alpha = alpha; a.beta = beta;
   a.C = C; a.Cin = Cin ? Cin : C; a.cstride = cstride; a.ldc = ldc; a.z = z; a.skip = skip;
-  // 32-wide tiles in a dimension below 256 whose last 64-wide tile would be at
-  // most half full (there the padding is a large share of the work; from 256 on
-  // it is at most 11%, and 64-wide tiles run faster than that)
-  const bool fr1 = m < 256 && m % 64 != 0 && m % 64 <= 32;
-  const bool fc1 = n < 256 && n % 64 != 0 && n % 64 <= 32;
+  // the tuning table's shape for (m, n, k), else the built-in one
+  int c = tune_lookup(TUNE_GEMM_TILE, m, n, k);
+  if (c == 0) c = builtin_tile(m, n);
+  const bool fr1 = c == TILE_32x32 || c == TILE_32x64;
+  const bool fc1 = c == TILE_32x32 || c == TILE_64x32;
   const int tm = fr1 ? 32 : 64, tn = fc1 ? 32 : 64;
   const dim3 grid(((m + tm - 1) / tm) * ((n + tn - 1) / tn), B);
   const bool sa = A.s != nullptr, sb = Bo.s != nullptr;
@@ -711,15 +802,24 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
                           cx<double>* ws, const cx<double>* A_copy = nullptr,
                           const int32_t* skip = nullptr) {
   const int64_t s = (int64_t)n * n;
-  if (n <= 16) hipLaunchKernelGGL((binv_kernel<16, 256>), dim3(B), dim3(256), 0, st, A, n, s, status, skip);
+  // the tuning table's form for n, else the built-in one
+  int form = 0;
+  if (n <= GJ_NMAX_LARGE) {
+    form = tune_lookup(TUNE_INVERSE, n, 0, 0);
+    if (form == 0 || (form == INV_LDS && n > 96)) form = builtin_inverse(n);
+  }
+  if (form == INV_LDS) {
+    if (n <= 64)
+      hipLaunchKernelGGL((binv_lds_kernel<64, 256>), dim3(B), dim3(256), 0, st, A, n, s, status, skip);
+    else
+      hipLaunchKernelGGL((binv_lds_kernel<96, 512>), dim3(B), dim3(512), 0, st, A, n, s, status, skip);
+  }
+  else if (n <= 16) hipLaunchKernelGGL((binv_kernel<16, 256>), dim3(B), dim3(256), 0, st, A, n, s, status, skip);
   else if (n <= 32) hipLaunchKernelGGL((binv_kernel<32, 256>), dim3(B), dim3(256), 0, st, A, n, s, status, skip);
   else if (n <= 64)  // 8 rows per thread and 71 registers: three workgroups per CU
     hipLaunchKernelGGL((binv_kernel<64, 512, 4>), dim3(B), dim3(512), 0, st, A, n, s, status, skip);
-  // 65..96: blocked Gauss-Jordan in LDS with matrix-core rank-4 updates (the
-  // scalar register-resident sweep above is latency-bound at n pivot steps
-  // with two workgroup barriers each)
-  else if (n <= 96)
-    hipLaunchKernelGGL((binv_lds_kernel<96, 512>), dim3(B), dim3(512), 0, st, A, n, s, status, skip);
+  else if (n <= 96)  // registers capped at 128: two workgroups per CU
+    hipLaunchKernelGGL((binv_kernel<96, 512, 4>), dim3(B), dim3(512), 0, st, A, n, s, status, skip);
   else if (n <= GJ_NMAX_LARGE)
     hipLaunchKernelGGL((binv_kernel<GJ_NMAX_LARGE, 1024>), dim3(B), dim3(1024), 0, st, A, n, s, status, skip);
   else if (n <= BLOCK_NMAX) {

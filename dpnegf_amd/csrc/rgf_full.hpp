@@ -15,6 +15,23 @@ int64_t inverse_workspace(int n, int B);
 // In-place inverse of B contiguous n x n blocks, 1 <= n <= GJ_NMAX_LARGE.
 hipError_t zinv_batched(hipStream_t st, cx<double>* A, int n, int B, int32_t* status);
 
+// Process-wide tuning table read by the batched product and inverse drivers
+// when they pick a kernel shape.  TUNE_GEMM_TILE keys (m, n, k), choices
+// TILE_32x32 .. TILE_64x64 (output tile rows x columns per workgroup);
+// TUNE_INVERSE keys (n, 0, 0), choices INV_REGISTER (register-resident
+// Gauss-Jordan, n <= 128) and INV_LDS (blocked Gauss-Jordan in LDS, n <= 96).
+// Choice 0 clears an entry.  With recording on, every (kind, m, n, k) a driver
+// looks up is collected (in first-seen order) for the autotuner to time.
+enum { TUNE_GEMM_TILE = 0, TUNE_INVERSE = 1 };
+enum { TILE_32x32 = 1, TILE_32x64 = 2, TILE_64x32 = 3, TILE_64x64 = 4 };
+enum { INV_REGISTER = 1, INV_LDS = 2 };
+int tune_set(int kind, int m, int n, int k, int choice);  // 0, or -1 for a bad argument
+int tune_get(int kind, int m, int n, int k);              // 0 when not set
+int tune_builtin(int kind, int m, int n, int k);          // the choice without an entry (0: bad key)
+void tune_clear();
+void tune_record(bool on);                                // on: also forgets earlier shapes
+int tune_recorded(int32_t* out, int cap);  // writes min(count, cap) x 4 ints, returns count
+
 // Operand value (i, k) = op(h[b] - z_b s[b]) with s optional, row-major ld.
 struct Opnd {
   const cx<double>* h;

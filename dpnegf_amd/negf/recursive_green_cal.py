@@ -261,7 +261,8 @@ def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", sta
     ``chunk``: points per call of the block-operation sweep (blocks above 64);
     by default ``autotune.default_chunk`` (DEFAULT_CHUNK, fewer when that many
     would not fit in half the free device memory), or
-    ``autotune.blocked_chunk``'s pick when ``DPNEGF_AUTOTUNE=1``."""
+    ``autotune.blocked_chunk``'s pick when ``DPNEGF_AUTOTUNE=1`` (which also
+    tunes the sweep's kernel shapes, ``autotune.tune_blocked``)."""
     if not isinstance(systems, BTDSystems):
         raise TypeError("systems must be a BTDSystems")
     dev = systems.device
@@ -284,6 +285,8 @@ def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", sta
     if chunk is None:
         chunk = (autotune.blocked_chunk(systems, P) if autotune.enabled()
                  else autotune.default_chunk(systems))
+    if autotune.enabled():
+        autotune.tune_blocked(systems, min(chunk, P))
     for s in range(systems.nsys):
         pts = (idx == s).nonzero().reshape(-1)
         if pts.numel() == 0:

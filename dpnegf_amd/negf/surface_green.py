@@ -18,6 +18,7 @@ import numpy as np
 import torch
 
 from .. import _native as nat
+from . import autotune
 from ._btd import C128
 
 log = logging.getLogger(__name__)
@@ -111,6 +112,8 @@ def self_energy_batched(leads, energies, eta_lead, sys_idx=None, E_ref=0.0, Bulk
     status = torch.zeros(P, dtype=torch.int32, device=device)
     resid = torch.zeros(P, dtype=torch.float64, device=device)
     chunk = points_per_call(n, a, P, device)
+    if n > nat.LEAD_NMAX and autotune.enabled():
+        _tune_decimation(arena, stride, n, a, mode, e, eta_lead, stop, ortho, min(chunk, P), device)
     eps = torch.empty(nat.load().dpnegf_lead_self_energy_workspace_elems(n, a, chunk),
                       dtype=C128, device=device)
     for c0 in range(0, P, chunk):
@@ -124,6 +127,27 @@ def self_energy_batched(leads, energies, eta_lead, sys_idx=None, E_ref=0.0, Bulk
     if raise_on_error:
         check_status(res, e)
     return res
+
+
+def _tune_decimation(arena, stride, n, a, mode, e, eta_lead, stop, ortho, B, device, sample=4):
+    """Kernel shapes of the batched decimation (``autotune.tune_lead``),
+    recorded on the first ``sample`` points of system 0 into scratch outputs."""
+    m = min(sample, e.shape[0])
+
+    def run_sample():
+        ws = torch.empty(nat.load().dpnegf_lead_self_energy_workspace_elems(n, a, m), dtype=C128,
+                         device=device)
+        gs = torch.empty(m, n, n, dtype=C128, device=device)
+        sig = torch.empty(m, a, a, dtype=C128, device=device)
+        it = torch.zeros(m, dtype=torch.int32, device=device)
+        st = torch.zeros(m, dtype=torch.int32, device=device)
+        rs = torch.zeros(m, dtype=torch.float64, device=device)
+        nat.call("dpnegf_lead_self_energy", nat.stream_ptr(device), n, a, mode, nat.ptr(arena),
+                 stride, 1, None, m, nat.ptr(e[:m].contiguous()), float(eta_lead), float(stop),
+                 nat.ptr(ws), nat.ptr(gs), nat.ptr(sig), int(ortho), nat.ptr(it), nat.ptr(st),
+                 nat.ptr(rs))
+
+    autotune.tune_lead(run_sample, n, B, device)
 
 
 def check_status(res, e):

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define DPNEGF_ABI_VERSION 3
+#define DPNEGF_ABI_VERSION 4
 
 #define DPNEGF_OK 0
 #define DPNEGF_ERR_ARG (-1)
@@ -132,6 +132,40 @@ int64_t dpnegf_rgf_transmission_blocked_workspace_elems(int K, const int32_t* h_
  * The reference has no counterpart: its runner splits the energy grid into
  * chunks of a user-set size (NEGF.py:602-631). */
 int64_t dpnegf_autotune_query(int K, const int32_t* h_bsz, int64_t mem_bytes);
+
+/* Kernel-shape tuning table, process-wide, read by the block-operation path
+ * (dpnegf_rgf_transmission_blocked, dpnegf_rgf_full, the batched decimation of
+ * dpnegf_lead_self_energy above 48 orbitals, dpnegf_zgemm_batched,
+ * dpnegf_zinv_batched) each time it launches a product or an inverse.
+ *  - DPNEGF_TUNE_GEMM_TILE, key (m, n, k) of a batched product: choice
+ *    DPNEGF_TILE_32x32 / _32x64 / _64x32 / _64x64, the output tile (rows x
+ *    columns) of one workgroup.
+ *  - DPNEGF_TUNE_INVERSE, key (n, 0, 0) of a batched inverse: choice
+ *    DPNEGF_INV_REGISTER (register-resident Gauss-Jordan, n <= 128) or
+ *    DPNEGF_INV_LDS (blocked Gauss-Jordan in LDS with matrix-core updates,
+ *    n <= 96).  Inverses above 128 recurse on their halves, keyed by the half.
+ * Choice 0 removes an entry; keys without an entry keep the built-in choice.
+ * Every choice gives the same result to rounding.  Set entries before a call
+ * is enqueued or captured, not while another thread launches.  While
+ * recording is on (dpnegf_tune_record(1), which also forgets earlier shapes)
+ * the library collects each key it looks up; dpnegf_tune_recorded writes
+ * min(count, cap) records of 4 int32 (kind, m, n, k) and returns the count.
+ * The reference has no counterpart: torch picks its library kernels. */
+#define DPNEGF_TUNE_GEMM_TILE 0
+#define DPNEGF_TUNE_INVERSE 1
+#define DPNEGF_TILE_32x32 1
+#define DPNEGF_TILE_32x64 2
+#define DPNEGF_TILE_64x32 3
+#define DPNEGF_TILE_64x64 4
+#define DPNEGF_INV_REGISTER 1
+#define DPNEGF_INV_LDS 2
+int dpnegf_tune_set(int kind, int m, int n, int k, int choice);
+int dpnegf_tune_get(int kind, int m, int n, int k);
+/* The choice a key gets without a table entry; 0 for a key no driver can use. */
+int dpnegf_tune_builtin(int kind, int m, int n, int k);
+void dpnegf_tune_clear(void);
+void dpnegf_tune_record(int on);
+int dpnegf_tune_recorded(int32_t* out, int cap);
 
 /* Transmission of B energies of one system for block sizes up to
  * DPNEGF_FULL_NMAX (the fused kernel stops at 64): the same sweep driven block

@@ -259,3 +259,45 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_tuning_table_without_a_device():
+    """The kernel-shape table is host state: entries are validated, read back,
+    removed by choice 0 and cleared; nothing is recorded while recording is off."""
+    lib = nat.load()
+    lib.dpnegf_tune_clear()
+    try:
+        assert lib.dpnegf_tune_set(nat.TUNE_GEMM_TILE, 96, 168, 120, nat.TILE_32x64) == 0
+        assert lib.dpnegf_tune_get(nat.TUNE_GEMM_TILE, 96, 168, 120) == nat.TILE_32x64
+        assert lib.dpnegf_tune_get(nat.TUNE_GEMM_TILE, 96, 168, 121) == 0
+        assert lib.dpnegf_tune_set(nat.TUNE_GEMM_TILE, 96, 168, 120, 5) != 0
+        assert lib.dpnegf_tune_set(nat.TUNE_GEMM_TILE, 0, 168, 120, 1) != 0
+        assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 90, 0, 0, nat.INV_LDS) == 0
+        assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 97, 0, 0, nat.INV_LDS) != 0
+        assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 129, 0, 0, nat.INV_REGISTER) != 0
+        assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 90, 1, 0, nat.INV_LDS) != 0
+        assert lib.dpnegf_tune_set(2, 1, 1, 1, 1) != 0
+        assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 90, 0, 0, 0) == 0
+        assert lib.dpnegf_tune_get(nat.TUNE_INVERSE, 90, 0, 0) == 0
+        lib.dpnegf_tune_clear()
+        assert lib.dpnegf_tune_get(nat.TUNE_GEMM_TILE, 96, 168, 120) == 0
+        lib.dpnegf_tune_record(1)
+        lib.dpnegf_tune_record(0)
+        assert lib.dpnegf_tune_recorded(None, 0) == 0
+    finally:
+        lib.dpnegf_tune_clear()
+
+
+def test_tuning_builtin_choices():
+    """The built-in choice the autotuner compares against: 32-wide tiles in a
+    dimension below 256 whose last 64-wide tile is at most half full, the LDS
+    inverse for 65..96 orbitals."""
+    lib = nat.load()
+    assert lib.dpnegf_tune_builtin(nat.TUNE_GEMM_TILE, 96, 96, 96) == nat.TILE_32x32
+    assert lib.dpnegf_tune_builtin(nat.TUNE_GEMM_TILE, 96, 168, 96) == nat.TILE_32x64
+    assert lib.dpnegf_tune_builtin(nat.TUNE_GEMM_TILE, 120, 144, 8) == nat.TILE_64x32
+    assert lib.dpnegf_tune_builtin(nat.TUNE_GEMM_TILE, 480, 128, 64) == nat.TILE_64x64
+    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 64, 0, 0) == nat.INV_REGISTER
+    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 65, 0, 0) == nat.INV_LDS
+    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 97, 0, 0) == nat.INV_REGISTER
+    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 129, 0, 0) == 0
