@@ -80,16 +80,21 @@ def ls_flops_per_point(n, iters):
 
 
 def rgf_flops_per_point(sizes, nL, nR):
-    """Conventional real flops of the fused transmission sweep per point
-    (8 per complex multiply-add, 8 n^3 per complex inverse)."""
-    n0 = sizes[0]
+    """Conventional real flops of the transmission sweep per point (8 per
+    complex multiply-add, 8 n^3 per complex inverse): per block Y = g U, the
+    carried S = S Y (nL rows Gamma_L reaches; none for the first block, where
+    S = Y), L Y and the inverse; then the corner Q = S g_{K-1} and the trace."""
+    n0, nK = sizes[0], sizes[-1]
     f = 8.0 * n0 ** 3  # g_0
     for q in range(len(sizes) - 1):
         nq, n1 = sizes[q], sizes[q + 1]
-        f += 8.0 * (n0 + nq) * nq * n1  # [P; g] U
-        f += 8.0 * n1 * nq * n1         # L Y
-        f += 8.0 * n1 ** 3              # inverse
-        f += 8.0 * n0 * n1 * n1         # P g
+        f += 8.0 * nq * nq * n1          # Y = g U
+        if q > 0:
+            f += 8.0 * nL * nq * n1      # S Y
+        f += 8.0 * n1 * nq * n1          # L Y
+        f += 8.0 * n1 ** 3               # inverse
+    if len(sizes) > 1:
+        f += 8.0 * nL * nK * nR          # Q = S g_{K-1} corner
     f += 8.0 * nL * nR * nR + 8.0 * nL * nL * nR  # trace
     return f
 

@@ -1070,8 +1070,8 @@ int64_t rgf_transmission_ops_workspace(int K, const int32_t* bsz, int B) {
 }
 
 // Transmission for any block size (<= BLOCK_NMAX) by batched block operations:
-// the same sweep as the fused kernel (P = G^L_{0,q} carried forward), then
-// T = Re Tr[Gamma_L Q Gamma_R Q^H] with Q the corner of P.
+// the same sweep as the fused kernel (S_q = G^L_{0,q} U_q carried forward), then
+// T = Re Tr[Gamma_L Q Gamma_R Q^H] with Q the corner of G_{0,K-1}.
 hipError_t rgf_transmission_ops(const RgfOpsArgs& a, hipStream_t st) {
   const int K = a.K, B = a.B;
   const std::vector<int> n(a.bsz, a.bsz + K);
@@ -1111,33 +1111,49 @@ hipError_t rgf_transmission_ops(const RgfOpsArgs& a, hipStream_t st) {
                        right ? a.sigR : nullptr, a.nR, nullptr);
     return hipGetLastError();
   };
-  const int n0 = n[0];
+  const int n0 = n[0], nK = n[K - 1], nL = a.nL, nR = a.nR, c0 = nK - nR;
   CHK(build(0, true, K == 1));
   CHK(inverse(st, g, n0, B, a.status, iws));
-  CHK(hipMemcpyAsync(P, g, sizeof(cx<double>) * B * n0 * n0, hipMemcpyDeviceToDevice, st));
+  // S_q = G^L_{0,q} U_q = S_{q-1} Y_q with Y_q = g_q U_q (S_0 = Y_0), batch
+  // stride n0 n_{q+1} and ld n_{q+1}; only the first nL rows, the rows Gamma_L
+  // reaches, are carried
+  cx<double>* S = P;
+  cx<double>* S2 = P2;
   for (int q = 0; q + 1 < K; ++q) {
     const int nq = n[q], n1 = n[q + 1];
     CHK(gemm(st, B, nq, n1, nq, mat(g, nq, (int64_t)nq * nq), Uop(q), one, zero, Y, (int64_t)nq * n1,
              n1, a.z));
-    CHK(gemm(st, B, n0, n1, nq, mat(P, nq, (int64_t)n0 * nq), Uop(q), one, zero, P2,
-             (int64_t)n0 * n1, n1, a.z));
+    if (q == 0) {
+      std::swap(S, Y);  // S_0 = Y_0 (first nL rows read through ld n1)
+    } else {
+      CHK(gemm(st, B, nL, n1, nq, mat(S, nq, (int64_t)n0 * nq), mat(Y, n1, (int64_t)nq * n1), one,
+               zero, S2, (int64_t)n0 * n1, n1, a.z));
+      std::swap(S, S2);
+    }
+    const cx<double>* Yq = q == 0 ? S : Y;
     CHK(build(q + 1, false, q + 1 == K - 1));
-    CHK(gemm(st, B, n1, n1, nq, Lop(q), mat(Y, n1, (int64_t)nq * n1), mone, one, g, (int64_t)n1 * n1,
-             n1, a.z));
+    CHK(gemm(st, B, n1, n1, nq, Lop(q), mat(Yq, n1, (int64_t)nq * n1), mone, one, g,
+             (int64_t)n1 * n1, n1, a.z));
     CHK(inverse(st, g, n1, B, a.status, iws));
-    CHK(gemm(st, B, n0, n1, n1, mat(P2, n1, (int64_t)n0 * n1), mat(g, n1, (int64_t)n1 * n1), one,
-             zero, P, (int64_t)n0 * n1, n1, a.z));
   }
-  const int nK = n[K - 1], nL = a.nL, nR = a.nR, c0 = nK - nR;
+  // Q = G_{0,K-1}[0:nL, c0:c0+nR] = S_{K-2} g_{K-1}[:, c0:] (g_0's corner for K = 1)
+  const cx<double>* Q = g + c0;
+  int ldq = nK;
+  int64_t sq = (int64_t)nK * nK;
+  if (K > 1) {  // S_{K-2}: batch stride n0 nK, ld nK
+    CHK(gemm(st, B, nL, nR, nK, mat(S, nK, (int64_t)n0 * nK), mat(g + c0, nK, (int64_t)nK * nK),
+             one, zero, S2, (int64_t)nL * nR, nR, a.z));
+    Q = S2; ldq = nR; sq = (int64_t)nL * nR;
+  }
   hipLaunchKernelGGL(bgamma_pad_kernel, dim3(grid_of((int64_t)B * nL * nL)), dim3(256), 0, st, gamL,
                      a.sigL, nL, nL, B, 0, (const double*)nullptr, 1.0);
   hipLaunchKernelGGL(bgamma_pad_kernel, dim3(grid_of((int64_t)B * nR * nR)), dim3(256), 0, st, gamR,
                      a.sigR, nR, nR, B, 0, (const double*)nullptr, 1.0);
   CHK(hipGetLastError());
   // V = Q Gamma_R   [nL x nR];  W = V Q^H  [nL x nL];  T = Re Tr[Gamma_L W]
-  CHK(gemm(st, B, nL, nR, nR, mat(P + c0, nK, (int64_t)n0 * nK), mat(gamR, nR, (int64_t)nR * nR), one,
+  CHK(gemm(st, B, nL, nR, nR, mat(Q, ldq, sq), mat(gamR, nR, (int64_t)nR * nR), one,
            zero, V, (int64_t)nL * nR, nR, a.z));
-  CHK(gemm(st, B, nL, nL, nR, mat(V, nR, (int64_t)nL * nR), mat(P + c0, nK, (int64_t)n0 * nK, OP_H),
+  CHK(gemm(st, B, nL, nL, nR, mat(V, nR, (int64_t)nL * nR), mat(Q, ldq, sq, OP_H),
            one, zero, W, (int64_t)nL * nL, nL, a.z));
   hipLaunchKernelGGL(btrace_re_kernel, dim3(B), dim3(256), 0, st, a.T, gamL, (int64_t)nL * nL, W,
                      (int64_t)nL * nL, nL, nL);

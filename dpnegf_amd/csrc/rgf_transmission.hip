@@ -3,16 +3,20 @@
 // Replaces the forward loop of recursive_gf_cal (dpnegf/negf/recursive_green_cal.py
 // :198-207), the g_trans product of its backward loop (:223-228) and the trace of
 // DeviceProperty._cal_tc_ (dpnegf/negf/device_property.py:383-404).  Instead of
-// storing every g_q and walking back, the sweep carries P_q = G^L_{0,q}:
+// storing every g_q and walking back, the sweep carries S_q = G^L_{0,q} U_q,
+// which needs no product of its own: with Y_q = g_q U_q, formed anyway for the
+// Dyson term,
 //
-//   g_0 = (-M_0)^{-1},  P_0 = g_0
-//   Y   = g_q U_q,      P  <- P U_q                     (one stacked product)
-//   g_{q+1} = (-M_{q+1} - L_q Y)^{-1}
-//   P_{q+1} = P U_q g_{q+1}
+//   g_0 = (-M_0)^{-1},  S_{-1} = I
+//   Y_q = g_q U_q
+//   S_q = S_{q-1} Y_q,  g_{q+1} = (-M_{q+1} - L_q Y_q)^{-1}   (one stacked product)
 //
-// so P_{K-1} = g_0 U_0 g_1 ... U_{K-2} g_{K-1} = G_{0,K-1}, the product the
-// reference builds right-to-left.  Then T = Re Tr[Gamma_L P Gamma_R P^dagger],
-// Gamma = i(Sigma - Sigma^dagger) formed on the fly from the self-energies.
+// so S_{K-2} g_{K-1} = g_0 U_0 g_1 ... U_{K-2} g_{K-1} = G_{0,K-1}, the product
+// the reference builds right-to-left: three block products per block instead
+// of the four of carrying G^L_{0,q} itself (P U_q and P g_{q+1}).  Only the
+// corner Q = G_{0,K-1}[0:nL, K-1 corner] is formed at the end, and
+// T = Re Tr[Gamma_L Q Gamma_R Q^dagger], Gamma = i(Sigma - Sigma^dagger) formed
+// on the fly from the self-energies.
 //
 // Arena layout of one system (complex128, row-major), for q = 0..K-1:
 //   hd_q [n_q x n_q], sd_q [n_q x n_q], and for q < K-1:
@@ -76,7 +80,7 @@ void rgf_transmission_kernel(RgfArgs a) {
   const cx<double>* sR = a.sigR + (int64_t)pt * a.nR * a.nR;
   const int nL = a.nL, nR = a.nR;
 
-  cx<R>* P = PG;
+  cx<R>* S = PG;  // S_{q-1}, n0 rows
   cx<R>* g = PG + n0 * LD;
 
   // M(i,k) = H - z S (+ Sigma corners), in double before rounding to R
@@ -159,10 +163,9 @@ void rgf_transmission_kernel(RgfArgs a) {
   }
   __syncthreads();
   int sing = inverse_g(n0);
-  __syncthreads();
   for (int e = threadIdx.x; e < n0 * n0; e += NTH) {
     const int i = e / n0, k = e % n0;
-    P[i * LD + k] = g[i * LD + k];
+    S[i * LD + k] = cx<R>{i == k ? R(1) : R(0), R(0)};
   }
   __syncthreads();
 
@@ -178,53 +181,62 @@ void rgf_transmission_kernel(RgfArgs a) {
     const cx<double>* sd1 = hd1 + (int64_t)n1 * n1;
     const bool last = (q + 1 == K - 1);
 
-    // [P; g] <- [P; g] (U_q - z SU_q)
-    auto a1 = [&](int i, int k) { return PG[i * LD + k]; };
+    // Y = g_q (U_q - z SU_q), in place in g
+    auto a1 = [&](int i, int k) { return g[i * LD + k]; };
     auto b1 = [&](int k, int j) {
       cx<double> u = hu[k * n1 + j];
       if (!ORTHO) u = u - zd * su[k * n1 + j];
       return cvt<R>(u);
     };
-    auto e1 = [&](int i, int j, cx<R> v) { PG[i * LD + j] = v; };
-    if constexpr (FIXED) wg_gemm_fixed<R, NW, 2 * NMAX, NMAX, NMAX, false, true>(a1, b1, e1);
-    else wg_gemm<R, NW, MAXT_STACK>(n0 + nq, n1, nq, a1, b1, e1);
+    auto e1 = [&](int i, int j, cx<R> v) { g[i * LD + j] = v; };
+    if constexpr (FIXED) wg_gemm_fixed<R, NW, NMAX, NMAX, NMAX, false, true>(a1, b1, e1);
+    else wg_gemm<R, NW, MAXT_SQ>(nq, n1, nq, a1, b1, e1);
     __syncthreads();
-    // g <- -M_{q+1} - (L_q - z SL_q) Y
+    // [S; g] <- [S Y; -M_{q+1} - (L_q - z SL_q) Y]: rows below n0 read S from
+    // LDS, the others L_q through L2 (a 16-row tile never mixes the two when
+    // the blocks are multiples of 16)
     auto a2 = [&](int i, int k) {
-      cx<double> l = hl[i * nq + k];
-      if (!ORTHO) l = l - zd * sl[i * nq + k];
+      if (i < n0) return S[i * LD + k];
+      cx<double> l = hl[(i - n0) * nq + k];
+      if (!ORTHO) l = l - zd * sl[(i - n0) * nq + k];
       return cvt<R>(l);
     };
     auto b2 = [&](int k, int j) { return g[k * LD + j]; };
     auto e2 = [&](int i, int j, cx<R> v) {
-      g[i * LD + j] = cvt<R>(-(cvt<double>(v) + Mval(hd1, sd1, n1, i, j, false, last)));
+      if (i < n0) S[i * LD + j] = v;
+      else
+        g[(i - n0) * LD + j] =
+            cvt<R>(-(cvt<double>(v) + Mval(hd1, sd1, n1, i - n0, j, false, last)));
     };
-    if constexpr (FIXED) wg_gemm_fixed<R, NW, NMAX, NMAX, NMAX, true, false>(a2, b2, e2);
-    else wg_gemm<R, NW, MAXT_SQ>(n1, n1, nq, a2, b2, e2);
+    // every tile's A operand is loaded one k-step ahead (prefetching only the
+    // L_q tiles measured 78.8 against 71.1 ms at NMAX 48, with 244 B of
+    // register spills per lane against 64 B)
+    if constexpr (FIXED) wg_gemm_fixed<R, NW, 2 * NMAX, NMAX, NMAX, true, false>(a2, b2, e2);
+    else wg_gemm<R, NW, MAXT_STACK>(n0 + n1, n1, nq, a2, b2, e2);
     __syncthreads();
     sing |= inverse_g(n1);  // ends with a workgroup barrier
-    // P <- P g_{q+1}
-    auto a3 = [&](int i, int k) { return P[i * LD + k]; };
-    auto b3 = [&](int k, int j) { return g[k * LD + j]; };
-    auto e3 = [&](int i, int j, cx<R> v) { P[i * LD + j] = v; };
-    if constexpr (FIXED) wg_gemm_fixed<R, NW, NMAX, NMAX, NMAX, false, false>(a3, b3, e3);
-    else wg_gemm<R, NW, MAXT_SQ>(n0, n1, n1, a3, b3, e3);
-    __syncthreads();
     off = off1;
   }
 
-  // T = Re sum_{a,b} GammaL[a][b] (Q GammaR Q^dagger)[b][a],  Q = P[0:nL, c0:c0+nR]
+  // Q = (S_{K-2} g_{K-1})[0:nL, c0:c0+nR] = G_{0,K-1} corner, in place in S
+  // (S = I for K = 1); then
+  // T = Re sum_{a,b} GammaL[a][b] (Q GammaR Q^dagger)[b][a]
   const int c0 = nlast - nR;
+  wg_gemm<R, NW, MAXT_SQ>(
+      nL, nR, nlast, [&](int i, int k) { return S[i * LD + k]; },
+      [&](int k, int j) { return g[k * LD + c0 + j]; },
+      [&](int i, int j, cx<R> v) { S[i * LD + j] = v; });
+  __syncthreads();
   cx<R>* V = g;
   wg_gemm<R, NW, MAXT_SQ>(
-      nL, nR, nR, [&](int i, int k) { return P[i * LD + c0 + k]; },
+      nL, nR, nR, [&](int i, int k) { return S[i * LD + k]; },
       [&](int k, int j) { return cvt<R>(gamma_of(sR, nR, k, j)); },
       [&](int i, int j, cx<R> v) { V[i * LD + j] = v; });
   __syncthreads();
   double acc = 0.0;
   wg_gemm<R, NW, MAXT_SQ>(
       nL, nL, nR, [&](int i, int k) { return V[i * LD + k]; },
-      [&](int k, int j) { return conj(P[j * LD + c0 + k]); },
+      [&](int k, int j) { return conj(S[j * LD + k]); },
       [&](int i, int j, cx<R> x) {
         const cx<double> gl = gamma_of(sL, nL, j, i);
         acc += gl.re * (double)x.re - gl.im * (double)x.im;
