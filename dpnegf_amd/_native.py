@@ -10,7 +10,7 @@ import torch
 
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libdpnegf_amd.so")
 
-ABI_VERSION = 4  # DPNEGF_ABI_VERSION of include/dpnegf_amd.h this binding matches
+ABI_VERSION = 5  # DPNEGF_ABI_VERSION of include/dpnegf_amd.h this binding matches
 
 DPNEGF_FP64 = 0
 DPNEGF_MIXED = 1
@@ -45,7 +45,8 @@ _SIGS = {
     "dpnegf_last_error": ([], ctypes.c_char_p),
     "dpnegf_arena_elems": ([_i32, _vp], _i64),
     "dpnegf_rgf_transmission": ([_vp, _i32, _i32, _vp, _vp, _vp, _i64, _i32, _vp, _i32, _vp, _vp,
-                                 _i32, _vp, _i32, _i32, _vp, _vp], _i32),
+                                 _i32, _vp, _i32, _i32, _vp, _vp, _vp], _i32),
+    "dpnegf_rgf_transmission_workspace_elems": ([_i32, _vp, _i32], _i64),
     "dpnegf_lead_self_energy": ([_vp, _i32, _i32, _i32, _vp, _i64, _i32, _vp, _i32, _vp, _f64, _f64,
                                  _vp, _vp, _vp, _i32, _vp, _vp, _vp], _i32),
     "dpnegf_lead_self_energy_noalloc": ([_vp, _i32, _i32, _i32, _vp, _i64, _i32, _vp, _i32, _vp,

@@ -72,15 +72,22 @@ int64_t dpnegf_arena_elems(int K, const int32_t* h_bsz) {
   return arena_elems(K, h_bsz);
 }
 
+int64_t dpnegf_rgf_transmission_workspace_elems(int K, const int32_t* h_bsz, int P) {
+  int nmax = 0;
+  if (P < 0 || check_blocks(K, h_bsz, DPNEGF_TRANSMISSION_NMAX, &nmax)) return -1;
+  return (int64_t)P * h_bsz[0] * nmax;
+}
+
 int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h_bsz,
                             const int32_t* bsz, const void* arena, int64_t sys_stride, int nsys,
                             const int32_t* sys_idx, int P, const void* z, const void* sigL, int nL,
-                            const void* sigR, int nR, int ortho, double* T, int32_t* status) {
+                            const void* sigR, int nR, int ortho, double* T, void* work,
+                            int32_t* status) {
   int nmax = 0;
   if (int r = check_blocks(K, h_bsz, DPNEGF_TRANSMISSION_NMAX, &nmax)) return r;
   if (P < 0) return fail(DPNEGF_ERR_ARG, "P must be >= 0");
   if (P == 0) return 0;
-  if (!bsz || !arena || !z || !sigL || !sigR || !T)
+  if (!bsz || !arena || !z || !sigL || !sigR || !T || !work)
     return fail(DPNEGF_ERR_ARG, "null device pointer");
   if (nL < 1 || nL > h_bsz[0])
     return fail(DPNEGF_ERR_ARG, "left self-energy size %d not in [1, %d]", nL, h_bsz[0]);
@@ -99,7 +106,7 @@ int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h
   a.K = K; a.bsz = bsz; a.arena = (const cx<double>*)arena; a.sys_stride = sys_stride;
   a.sys_idx = sys_idx; a.nsys = nsys; a.P = P; a.z = (const cx<double>*)z;
   a.sigL = (const cx<double>*)sigL; a.nL = nL; a.sigR = (const cx<double>*)sigR; a.nR = nR;
-  a.T = T; a.status = status;
+  a.T = T; a.status = status; a.work = (cx<double>*)work;
   bool uniform = true;
   for (int q = 0; q < K; ++q) uniform = uniform && h_bsz[q] == nmax;
   hipError_t e;

@@ -46,6 +46,8 @@ struct RgfArgs {
   int nR;
   double* T;                 // device [P]
   int32_t* status;           // device [P] or null
+  cx<double>* work;          // device [P, n_0, nmax]: the carried S_q where a kernel keeps it
+                             // in global memory (rgf_transmission_in_global)
 };
 
 template <typename R>

@@ -30,9 +30,28 @@ namespace negf {
 // relative error: 1e-4 -> about 1e-8 before the FP64 rounding floor).
 constexpr double REFINE_TOL = 1e-4;
 
+// Where the carried S lives: in LDS next to g, or in the caller's work buffer
+// (global memory, L2-resident: 37 KB per point, 2.4 MB per XCD at three
+// workgroups per CU).  FP64 FIXED at NMAX = 48: 79 KB of LDS holds two
+// workgroups per CU, 41 KB three, and with three the serial pivot chains of
+// more points overlap (71.1 -> 64.1 ms on the W = 48, K = 128 ribbon sweep).
+// At NMAX = 64 the same move (133 -> 71 KB, two workgroups per CU instead of
+// one) measured 31.21 against 31.24 ms on the W = 64, K = 64 sweep: S stays in LDS.
+template <typename R, int NMAX, bool FIXED, bool REFINE>
+constexpr bool rgf_s_in_global() {
+  return FIXED && !REFINE && sizeof(R) == 8 && NMAX == 48;
+}
+// waves per SIMD the register budget is held to (workgroups per CU x NTH / 256)
+template <typename R, int NMAX, bool FIXED, bool REFINE, int NTH>
+constexpr int rgf_waves_per_eu() {
+  if (rgf_s_in_global<R, NMAX, FIXED, REFINE>()) return 3 * NTH / 256;
+  return (NMAX <= 48 && !REFINE ? 2 : 1) * NTH / 256;
+}
+
 // FIXED: every block has exactly NMAX rows (compile-time shapes for the inverse
 // and the products).  Up to NMAX = 48 two workgroups share a CU (LDS), so the
-// register budget is held to two waves per SIMD.
+// register budget is held to two waves per SIMD (three workgroups and three
+// waves with S in global memory).
 // REFINE (with R = double): every block inverse is computed in FP32 and
 // refined by one FP64 Newton step, X = X0 + X0 (I - A X0); a block whose FP32
 // residual exceeds REFINE_TOL is inverted in FP64 instead (status bit
@@ -40,8 +59,9 @@ constexpr double REFINE_TOL = 1e-4;
 // NTH: threads per workgroup (NTH / 64 waves share every product and update).
 template <typename R, int NMAX, bool ORTHO, bool FIXED, bool REFINE = false, int NTH = 256>
 __global__ __launch_bounds__(NTH)
-__attribute__((amdgpu_waves_per_eu(NMAX <= 48 && !REFINE ? 2 * NTH / 256 : NTH / 256)))
+__attribute__((amdgpu_waves_per_eu(rgf_waves_per_eu<R, NMAX, FIXED, REFINE, NTH>())))
 void rgf_transmission_kernel(RgfArgs a) {
+  constexpr bool SG = rgf_s_in_global<R, NMAX, FIXED, REFINE>();
   static_assert(!REFINE || (sizeof(R) == 8 && NMAX <= 48), "refinement: FP64 kernel, NMAX <= 48");
   constexpr int NW = NTH / WAVE;
   constexpr int LD = NMAX + 1;
@@ -49,7 +69,8 @@ void rgf_transmission_kernel(RgfArgs a) {
   constexpr int MAXT_STACK = (2 * NT + NW - 1) / NW;
   constexpr int MAXT_SQ = (NT + NW - 1) / NW;
   constexpr int MAXR = gj_maxr(NMAX, NTH);
-  __shared__ cx<R> PG[2 * NMAX * LD];
+  __shared__ cx<R> PG[(SG ? 1 : 2) * NMAX * LD];  // [S; g], or g alone
+  constexpr int SLD = SG ? NMAX : LD;             // leading dimension of S
   __shared__ GJScratch<R, NMAX> gjs;
   __shared__ GJBScratch<R, NMAX> gjb;
   __shared__ double red[NW > 8 ? NW : 8];
@@ -80,8 +101,9 @@ void rgf_transmission_kernel(RgfArgs a) {
   const cx<double>* sR = a.sigR + (int64_t)pt * a.nR * a.nR;
   const int nL = a.nL, nR = a.nR;
 
-  cx<R>* S = PG;  // S_{q-1}, n0 rows
-  cx<R>* g = PG + n0 * LD;
+  // S_{q-1}, n0 rows (FIXED: n0 = NMAX, so point pt's slot is work + pt n0 nmax)
+  cx<R>* S = SG ? (cx<R>*)a.work + (int64_t)pt * NMAX * NMAX : PG;
+  cx<R>* g = SG ? PG : PG + n0 * LD;
 
   // M(i,k) = H - z S (+ Sigma corners), in double before rounding to R
   auto Mval = [&](const cx<double>* h, const cx<double>* s, int nb, int i, int k, bool left,
@@ -165,7 +187,7 @@ void rgf_transmission_kernel(RgfArgs a) {
   int sing = inverse_g(n0);
   for (int e = threadIdx.x; e < n0 * n0; e += NTH) {
     const int i = e / n0, k = e % n0;
-    S[i * LD + k] = cx<R>{i == k ? R(1) : R(0), R(0)};
+    S[i * SLD + k] = cx<R>{i == k ? R(1) : R(0), R(0)};
   }
   __syncthreads();
 
@@ -196,14 +218,14 @@ void rgf_transmission_kernel(RgfArgs a) {
     // LDS, the others L_q through L2 (a 16-row tile never mixes the two when
     // the blocks are multiples of 16)
     auto a2 = [&](int i, int k) {
-      if (i < n0) return S[i * LD + k];
+      if (i < n0) return S[i * SLD + k];
       cx<double> l = hl[(i - n0) * nq + k];
       if (!ORTHO) l = l - zd * sl[(i - n0) * nq + k];
       return cvt<R>(l);
     };
     auto b2 = [&](int k, int j) { return g[k * LD + j]; };
     auto e2 = [&](int i, int j, cx<R> v) {
-      if (i < n0) S[i * LD + j] = v;
+      if (i < n0) S[i * SLD + j] = v;
       else
         g[(i - n0) * LD + j] =
             cvt<R>(-(cvt<double>(v) + Mval(hd1, sd1, n1, i - n0, j, false, last)));
@@ -211,7 +233,16 @@ void rgf_transmission_kernel(RgfArgs a) {
     // every tile's A operand is loaded one k-step ahead (prefetching only the
     // L_q tiles measured 78.8 against 71.1 ms at NMAX 48, with 244 B of
     // register spills per lane against 64 B)
-    if constexpr (FIXED) wg_gemm_fixed<R, NW, 2 * NMAX, NMAX, NMAX, true, false>(a2, b2, e2);
+    if constexpr (SG) {
+      // two 9-tile products instead of the stacked one: at three waves per
+      // SIMD (168 registers) the 18-tile form spills 476 B per lane, these 180 B
+      wg_gemm_fixed<R, NW, NMAX, NMAX, NMAX, true, false>(
+          [&](int i, int k) { return S[i * SLD + k]; }, b2,
+          [&](int i, int j, cx<R> v) { S[i * SLD + j] = v; });
+      wg_gemm_fixed<R, NW, NMAX, NMAX, NMAX, true, false, true, 1>(
+          [&](int i, int k) { return a2(i + n0, k); }, b2,
+          [&](int i, int j, cx<R> v) { e2(i + n0, j, v); });
+    } else if constexpr (FIXED) wg_gemm_fixed<R, NW, 2 * NMAX, NMAX, NMAX, true, false>(a2, b2, e2);
     else wg_gemm<R, NW, MAXT_STACK>(n0 + n1, n1, nq, a2, b2, e2);
     __syncthreads();
     sing |= inverse_g(n1);  // ends with a workgroup barrier
@@ -223,20 +254,20 @@ void rgf_transmission_kernel(RgfArgs a) {
   // T = Re sum_{a,b} GammaL[a][b] (Q GammaR Q^dagger)[b][a]
   const int c0 = nlast - nR;
   wg_gemm<R, NW, MAXT_SQ>(
-      nL, nR, nlast, [&](int i, int k) { return S[i * LD + k]; },
+      nL, nR, nlast, [&](int i, int k) { return S[i * SLD + k]; },
       [&](int k, int j) { return g[k * LD + c0 + j]; },
-      [&](int i, int j, cx<R> v) { S[i * LD + j] = v; });
+      [&](int i, int j, cx<R> v) { S[i * SLD + j] = v; });
   __syncthreads();
   cx<R>* V = g;
   wg_gemm<R, NW, MAXT_SQ>(
-      nL, nR, nR, [&](int i, int k) { return S[i * LD + k]; },
+      nL, nR, nR, [&](int i, int k) { return S[i * SLD + k]; },
       [&](int k, int j) { return cvt<R>(gamma_of(sR, nR, k, j)); },
       [&](int i, int j, cx<R> v) { V[i * LD + j] = v; });
   __syncthreads();
   double acc = 0.0;
   wg_gemm<R, NW, MAXT_SQ>(
       nL, nL, nR, [&](int i, int k) { return V[i * LD + k]; },
-      [&](int k, int j) { return conj(S[j * LD + k]); },
+      [&](int k, int j) { return conj(S[j * SLD + k]); },
       [&](int i, int j, cx<R> x) {
         const cx<double> gl = gamma_of(sL, nL, j, i);
         acc += gl.re * (double)x.re - gl.im * (double)x.im;

@@ -75,6 +75,9 @@ class TransmissionGraph:
         self.status = torch.zeros(self.P, dtype=torch.int32, device=dev)
         if self.blocked:
             self._prepare_blocked()
+        else:
+            self.work = torch.empty(nat.load().dpnegf_rgf_transmission_workspace_elems(
+                len(self.bs.sizes), self.bs._h_bsz, self.P), dtype=C128, device=dev)
         self._launch()  # eager run: validates arguments before capture
         torch.cuda.synchronize(dev)
         self.graph = torch.cuda.CUDAGraph()
@@ -139,7 +142,7 @@ class TransmissionGraph:
                  self.bs._h_bsz, nat.ptr(self.bs.bsz), nat.ptr(self.bs.arena), self.bs.stride,
                  self.bs.nsys, nat.ptr(self.sys_idx), self.P, nat.ptr(self.z), nat.ptr(sL),
                  sL.shape[-1], nat.ptr(sR), sR.shape[-1], int(self.bs.ortho), nat.ptr(self.T),
-                 nat.ptr(self.status))
+                 nat.ptr(self.work), nat.ptr(self.status))
 
     def replay(self):
         """Runs the captured launches; returns T [P] (the graph's own buffer)."""

@@ -272,10 +272,13 @@ def rgf_transmission(systems, z, sigL, sigR, sys_idx=None, precision="fp64", sta
     if max(systems.sizes) <= nat.TRANSMISSION_NMAX:
         prec = {"fp64": nat.DPNEGF_FP64, "mixed": nat.DPNEGF_MIXED,
                 "mixed_refined": nat.DPNEGF_MIXED_REFINED}[precision]
+        work = torch.empty(nat.load().dpnegf_rgf_transmission_workspace_elems(
+            len(systems.sizes), systems._h_bsz, P), dtype=C128, device=dev)
         nat.call("dpnegf_rgf_transmission", nat.stream_ptr(dev), prec, len(systems.sizes),
                  systems._h_bsz, nat.ptr(systems.bsz), nat.ptr(systems.arena), systems.stride,
                  systems.nsys, nat.ptr(sys_idx), P, nat.ptr(z), nat.ptr(sigL), sigL.shape[-1],
-                 nat.ptr(sigR), sigR.shape[-1], int(systems.ortho), nat.ptr(T), nat.ptr(status))
+                 nat.ptr(sigR), sigR.shape[-1], int(systems.ortho), nat.ptr(T), nat.ptr(work),
+                 nat.ptr(status))
         return T
     # larger blocks: block-operation sweep, one call per system and chunk of points
     if precision != "fp64":

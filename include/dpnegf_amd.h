@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define DPNEGF_ABI_VERSION 4
+#define DPNEGF_ABI_VERSION 5
 
 #define DPNEGF_OK 0
 #define DPNEGF_ERR_ARG (-1)
@@ -65,12 +65,20 @@ int64_t dpnegf_arena_elems(int K, const int32_t* h_bsz);
  *                a block whose FP32 inverse leaves a residual above 1e-4 is
  *                inverted in FP64 instead: DPNEGF_STATUS_REFINE_FALLBACK)
  *   T[p]       = Re Tr[Gamma_L G_{0,K-1} Gamma_R G_{0,K-1}^dagger]
+ *   work       = device buffer of dpnegf_rgf_transmission_workspace_elems(K, h_bsz, P)
+ *                complex elements (the sweep's carried G_{0,q} U_q of each point,
+ *                kept in global memory by the kernels whose LDS would otherwise
+ *                limit them to fewer workgroups per CU); contents on entry ignored
  *   status[p]  = DPNEGF_STATUS_* bits (may be null)
  * Block sizes up to DPNEGF_TRANSMISSION_NMAX. */
 int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h_bsz,
                             const int32_t* bsz, const void* arena, int64_t sys_stride, int nsys,
                             const int32_t* sys_idx, int P, const void* z, const void* sigL, int nL,
-                            const void* sigR, int nR, int ortho, double* T, int32_t* status);
+                            const void* sigR, int nR, int ortho, double* T, void* work,
+                            int32_t* status);
+/* P n_0 nmax: complex elements of dpnegf_rgf_transmission's work buffer (-1 on
+ * bad arguments).  No device work. */
+int64_t dpnegf_rgf_transmission_workspace_elems(int K, const int32_t* h_bsz, int P);
 
 /* Lopez-Sancho surface Green's function and lead self-energy for P points.
  * Replaces selfEnergy (dpnegf/negf/surface_green.py:9-91) and its decimation

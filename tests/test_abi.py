@@ -37,8 +37,16 @@ def test_argument_errors_are_reported_without_a_device():
     lib = nat.load()
     bsz = nat.host_i32([80])
     rc = lib.dpnegf_rgf_transmission(None, 0, 1, bsz, None, None, 0, 1, None, 1, None, None, 1,
-                                     None, 1, 1, None, None)
+                                     None, 1, 1, None, None, None)
     assert rc == -2 and b"exceeds" in lib.dpnegf_last_error()
+    # a missing work buffer is refused like any other null device pointer
+    ok = nat.host_i32([48, 48])
+    rc = lib.dpnegf_rgf_transmission(None, 0, 2, ok, 8, 8, 0, 1, None, 1, 8, 8, 48, 8, 48, 1, 8,
+                                     None, None)
+    assert rc == -1 and b"null" in lib.dpnegf_last_error()
+    assert lib.dpnegf_rgf_transmission_workspace_elems(2, ok, 10) == 10 * 48 * 48
+    assert lib.dpnegf_rgf_transmission_workspace_elems(2, nat.host_i32([40, 48]), 3) == 3 * 40 * 48
+    assert lib.dpnegf_rgf_transmission_workspace_elems(1, bsz, 3) == -1
     rc = lib.dpnegf_lead_self_energy(None, 0, 4, 0, None, 0, 1, None, 1, None, 1e-5, 1e-40,
                                      None, None, None, 1, None, None, None)
     assert rc == -1

@@ -135,11 +135,13 @@ def test_out_of_range_sys_idx_is_flagged_not_read(n):
     sig = s_ok.to(dev).contiguous()
     T = torch.empty(4, dtype=torch.float64, device=dev)
     st = torch.zeros(4, dtype=torch.int32, device=dev)
+    work = torch.empty(nat.load().dpnegf_rgf_transmission_workspace_elems(3, bs._h_bsz, 4),
+                       dtype=torch.complex128, device=dev)
     for idx in (good, bad):
         nat.call("dpnegf_rgf_transmission", nat.stream_ptr(dev), nat.DPNEGF_FP64, 3, bs._h_bsz,
                  nat.ptr(bs.bsz), nat.ptr(bs.arena), bs.stride, 1, nat.ptr(idx), 4,
                  nat.ptr((e + 1e-4j).contiguous()), nat.ptr(sig), n, nat.ptr(sig), n, int(bs.ortho),
-                 nat.ptr(T), nat.ptr(st))
+                 nat.ptr(T), nat.ptr(work), nat.ptr(st))
         torch.cuda.synchronize()
         if idx is good:
             T_ok = T.cpu().clone()
