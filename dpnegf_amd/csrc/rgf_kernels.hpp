@@ -46,8 +46,8 @@ struct RgfArgs {
   int nR;
   double* T;                 // device [P]
   int32_t* status;           // device [P] or null
-  cx<double>* work;          // device [P, n_0, nmax]: the carried S_q where a kernel keeps it
-                             // in global memory (rgf_transmission_in_global)
+  cx<double>* work;          // device [P, n_0, nmax]: per-point scratch of the kernels that keep
+                             // the carried S out of LDS (rgf_s_out_of_lds): the final corner Q
 };
 
 template <typename R>

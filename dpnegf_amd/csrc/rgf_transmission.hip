@@ -30,28 +30,41 @@ namespace negf {
 // relative error: 1e-4 -> about 1e-8 before the FP64 rounding floor).
 constexpr double REFINE_TOL = 1e-4;
 
-// Where the carried S lives: in LDS next to g, or in the caller's work buffer
-// (global memory, L2-resident: 37 KB per point, 2.4 MB per XCD at three
-// workgroups per CU).  FP64 FIXED at NMAX = 48: 79 KB of LDS holds two
-// workgroups per CU, 41 KB three, and with three the serial pivot chains of
-// more points overlap (71.1 -> 64.1 ms on the W = 48, K = 128 ribbon sweep).
-// At NMAX = 64 the same move (133 -> 71 KB, two workgroups per CU instead of
-// one) measured 31.21 against 31.24 ms on the W = 64, K = 64 sweep: S stays in LDS.
+// Where the carried S lives: in LDS next to g, or (SG) outside LDS, so that g
+// alone takes LDS.  FP64 FIXED at NMAX = 48: 79 KB of LDS holds two workgroups
+// per CU, 41 KB four (three-wave workgroups, below), and the serial pivot
+// chains of more points overlap.  SG keeps S transposed in registers (the
+// kernel body says how); held in the caller's work buffer instead it moved
+// about 80 GB per launch through the memory fabric at four workgroups per CU.
+// W = 48, K = 128 ribbon sweep: 71.1 ms (S in LDS, two workgroups per CU) ->
+// 64.1 (work buffer, three) -> 58.5 (work buffer, four) -> 55.3 ms (registers,
+// four, g^-1 left in pivot order).  At NMAX = 64, S in the work buffer (133 ->
+// 71 KB, two workgroups per CU instead of one) measured 31.21 against 31.24 ms
+// on the W = 64, K = 64 sweep: S stays in LDS there.
 template <typename R, int NMAX, bool FIXED, bool REFINE>
-constexpr bool rgf_s_in_global() {
+constexpr bool rgf_s_out_of_lds() {
   return FIXED && !REFINE && sizeof(R) == 8 && NMAX == 48;
 }
-// waves per SIMD the register budget is held to (workgroups per CU x NTH / 256)
+// Threads per workgroup.  With S out of LDS, three waves and 40.9 KB of LDS
+// (g, the panel multipliers and 16-bit pivot indices) let four workgroups
+// share a CU: four pivot chains overlap instead of three, and the 9-tile
+// products split 3/3/3 over the waves (with four waves the slowest also had
+// three).
+template <typename R, int NMAX, bool FIXED, bool REFINE>
+constexpr int rgf_threads() {
+  return rgf_s_out_of_lds<R, NMAX, FIXED, REFINE>() ? 192 : 256;
+}
+// waves per SIMD the register budget is held to (workgroups per CU x waves / 4)
 template <typename R, int NMAX, bool FIXED, bool REFINE, int NTH>
 constexpr int rgf_waves_per_eu() {
-  if (rgf_s_in_global<R, NMAX, FIXED, REFINE>()) return 3 * NTH / 256;
+  if (rgf_s_out_of_lds<R, NMAX, FIXED, REFINE>()) return 4 * (NTH / 64) / 4;
   return (NMAX <= 48 && !REFINE ? 2 : 1) * NTH / 256;
 }
 
 // FIXED: every block has exactly NMAX rows (compile-time shapes for the inverse
 // and the products).  Up to NMAX = 48 two workgroups share a CU (LDS), so the
-// register budget is held to two waves per SIMD (three workgroups and three
-// waves with S in global memory).
+// register budget is held to two waves per SIMD (four three-wave workgroups
+// and three waves per SIMD with S in global memory).
 // REFINE (with R = double): every block inverse is computed in FP32 and
 // refined by one FP64 Newton step, X = X0 + X0 (I - A X0); a block whose FP32
 // residual exceeds REFINE_TOL is inverted in FP64 instead (status bit
@@ -61,7 +74,7 @@ template <typename R, int NMAX, bool ORTHO, bool FIXED, bool REFINE = false, int
 __global__ __launch_bounds__(NTH)
 __attribute__((amdgpu_waves_per_eu(rgf_waves_per_eu<R, NMAX, FIXED, REFINE, NTH>())))
 void rgf_transmission_kernel(RgfArgs a) {
-  constexpr bool SG = rgf_s_in_global<R, NMAX, FIXED, REFINE>();
+  constexpr bool SG = rgf_s_out_of_lds<R, NMAX, FIXED, REFINE>();
   static_assert(!REFINE || (sizeof(R) == 8 && NMAX <= 48), "refinement: FP64 kernel, NMAX <= 48");
   constexpr int NW = NTH / WAVE;
   constexpr int LD = NMAX + 1;
@@ -73,7 +86,7 @@ void rgf_transmission_kernel(RgfArgs a) {
   constexpr int SLD = SG ? NMAX : LD;             // leading dimension of S
   __shared__ GJScratch<R, NMAX> gjs;
   __shared__ GJBScratch<R, NMAX> gjb;
-  __shared__ double red[NW > 8 ? NW : 8];
+  __shared__ double red[NW];
   // FP32 copy / inverse of the block and the FP64 residual (refined variant only)
   constexpr int NR = REFINE ? NMAX * LD : 1;
   __shared__ cx<float> XF[NR];
@@ -104,6 +117,19 @@ void rgf_transmission_kernel(RgfArgs a) {
   // S_{q-1}, n0 rows (FIXED: n0 = NMAX, so point pt's slot is work + pt n0 nmax)
   cx<R>* S = SG ? (cx<R>*)a.work + (int64_t)pt * NMAX * NMAX : PG;
   cx<R>* g = SG ? PG : PG + n0 * LD;
+  // SG: the carried S is held transposed in registers.  Wave w keeps column
+  // block w of S^T (rows 16 I + lg + 4 r of tile I in element r, the
+  // accumulator layout of the f64 matrix-core instruction), which is exactly
+  // the B operand of S^T <- Y^T S^T for k-step 4 I + r: the product needs no
+  // data movement, and S never reaches LDS or memory (the work buffer holds
+  // only the final corner Q).  Held in global memory instead, S moved about
+  // 80 GB per launch through the memory fabric at four workgroups per CU.
+  typedef typename mfma<R>::v4 v4;
+  constexpr int NB = NMAX / 16;
+  static_assert(!SG || NW == NB, "register S^T: one wave per 16-column block");
+  const int li = threadIdx.x & 15, lg = (threadIdx.x & (WAVE - 1)) >> 4;
+  const int wv = threadIdx.x / WAVE;
+  v4 str[SG ? NB : 1], sti[SG ? NB : 1];
 
   // M(i,k) = H - z S (+ Sigma corners), in double before rounding to R
   auto Mval = [&](const cx<double>* h, const cx<double>* s, int nb, int i, int k, bool left,
@@ -134,9 +160,13 @@ void rgf_transmission_kernel(RgfArgs a) {
           n, [&](int i, int k) { return g[i * LD + k]; },
           [&](int i, int k, cx<R> v) { g[i * LD + k] = v; }, &gjs);
   };
-  // g <- g^-1 in place; every thread calls it, and it ends with a barrier
+  // g <- g^-1 in place; every thread calls it, and it ends with a barrier.
+  // SG: g^-1 stays in pivot order, g^-1[i][k] = g[piv[i]][pinv[k]], and the
+  // consumers (the product Y = g U and the final corner) read through the maps.
   auto inverse_g = [&](int n) __attribute__((always_inline)) -> int {
-    if constexpr (!REFINE) {
+    if constexpr (SG) {
+      return wg_gj_inverse_pivoted<R, NTH, NMAX, LD>(g, &gjb);
+    } else if constexpr (!REFINE) {
       return inverse_full(n);
     } else {
       for (int e = threadIdx.x; e < n * n; e += NTH) {
@@ -185,11 +215,21 @@ void rgf_transmission_kernel(RgfArgs a) {
   }
   __syncthreads();
   int sing = inverse_g(n0);
-  for (int e = threadIdx.x; e < n0 * n0; e += NTH) {
-    const int i = e / n0, k = e % n0;
-    S[i * SLD + k] = cx<R>{i == k ? R(1) : R(0), R(0)};
+  if constexpr (SG) {
+#pragma unroll
+    for (int I = 0; I < NB; ++I)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        str[I][r] = (16 * I + lg + 4 * r == 16 * wv + li) ? R(1) : R(0);
+        sti[I][r] = R(0);
+      }
+  } else {
+    for (int e = threadIdx.x; e < n0 * n0; e += NTH) {
+      const int i = e / n0, k = e % n0;
+      S[i * SLD + k] = cx<R>{i == k ? R(1) : R(0), R(0)};
+    }
+    __syncthreads();
   }
-  __syncthreads();
 
   int64_t off = 0;
   for (int q = 0; q < K - 1; ++q) {
@@ -203,14 +243,20 @@ void rgf_transmission_kernel(RgfArgs a) {
     const cx<double>* sd1 = hd1 + (int64_t)n1 * n1;
     const bool last = (q + 1 == K - 1);
 
-    // Y = g_q (U_q - z SU_q), in place in g
+    // Y = g_q (U_q - z SU_q), in place in g.  SG, with g_q in pivot order:
+    // Y[i] = Z[piv[i]] for Z = g (U with row m taken from row piv[m]), so U is
+    // read through piv and Z's row r is stored as Y's row pinv[r].
     auto a1 = [&](int i, int k) { return g[i * LD + k]; };
     auto b1 = [&](int k, int j) {
-      cx<double> u = hu[k * n1 + j];
-      if (!ORTHO) u = u - zd * su[k * n1 + j];
+      const int kr = SG ? (int)gjb.piv[k] : k;
+      cx<double> u = hu[kr * n1 + j];
+      if (!ORTHO) u = u - zd * su[kr * n1 + j];
       return cvt<R>(u);
     };
-    auto e1 = [&](int i, int j, cx<R> v) { g[i * LD + j] = v; };
+    auto e1 = [&](int i, int j, cx<R> v) {
+      const int ir = SG ? (int)gjb.pinv[i] : i;
+      g[ir * LD + j] = v;
+    };
     if constexpr (FIXED) wg_gemm_fixed<R, NW, NMAX, NMAX, NMAX, false, true>(a1, b1, e1);
     else wg_gemm<R, NW, MAXT_SQ>(nq, n1, nq, a1, b1, e1);
     __syncthreads();
@@ -234,11 +280,33 @@ void rgf_transmission_kernel(RgfArgs a) {
     // L_q tiles measured 78.8 against 71.1 ms at NMAX 48, with 244 B of
     // register spills per lane against 64 B)
     if constexpr (SG) {
-      // two 9-tile products instead of the stacked one: at three waves per
-      // SIMD (168 registers) the 18-tile form spills 476 B per lane, these 180 B
-      wg_gemm_fixed<R, NW, NMAX, NMAX, NMAX, true, false>(
-          [&](int i, int k) { return S[i * SLD + k]; }, b2,
-          [&](int i, int j, cx<R> v) { S[i * SLD + j] = v; });
+      // S^T <- Y^T S^T in registers (A operand Y^T read from LDS), then
+      // g <- -M_{q+1} - L_q Y, whose epilogue barrier follows every read of Y
+      v4 nr[NB], ni[NB];
+#pragma unroll
+      for (int I = 0; I < NB; ++I) {
+        nr[I] = v4{0, 0, 0, 0};
+        ni[I] = v4{0, 0, 0, 0};
+      }
+      // one base address; k-step and tile offsets are immediates
+      const cx<R>* yb = g + lg * LD + li;
+#pragma unroll
+      for (int kk = 0; kk < NMAX / 4; ++kk) {
+        const R br = str[kk >> 2][kk & 3], bi = sti[kk >> 2][kk & 3];
+#pragma unroll
+        for (int I = 0; I < NB; ++I) {
+          const cx<R> y = yb[4 * kk * LD + 16 * I];
+          nr[I] = mfma<R>::op(y.re, br, nr[I]);
+          nr[I] = mfma<R>::op(-y.im, bi, nr[I]);
+          ni[I] = mfma<R>::op(y.re, bi, ni[I]);
+          ni[I] = mfma<R>::op(y.im, br, ni[I]);
+        }
+      }
+#pragma unroll
+      for (int I = 0; I < NB; ++I) {
+        str[I] = nr[I];
+        sti[I] = ni[I];
+      }
       wg_gemm_fixed<R, NW, NMAX, NMAX, NMAX, true, false, true, 1>(
           [&](int i, int k) { return a2(i + n0, k); }, b2,
           [&](int i, int j, cx<R> v) { e2(i + n0, j, v); });
@@ -253,10 +321,37 @@ void rgf_transmission_kernel(RgfArgs a) {
   // (S = I for K = 1); then
   // T = Re sum_{a,b} GammaL[a][b] (Q GammaR Q^dagger)[b][a]
   const int c0 = nlast - nR;
-  wg_gemm<R, NW, MAXT_SQ>(
-      nL, nR, nlast, [&](int i, int k) { return S[i * SLD + k]; },
-      [&](int k, int j) { return g[k * LD + c0 + j]; },
-      [&](int i, int j, cx<R> v) { S[i * SLD + j] = v; });
+  if constexpr (SG) {
+    // Q^T = (g[:, c0:c0+nR])^T S^T: tile (J, wv) from the registers, written
+    // to the work buffer as Q
+#pragma unroll
+    for (int J = 0; J < NB; ++J) {
+      v4 qr = v4{0, 0, 0, 0}, qi = v4{0, 0, 0, 0};
+      const int jc = 16 * J + li;
+#pragma unroll
+      for (int kk = 0; kk < NMAX / 4; ++kk) {
+        const R br = str[kk >> 2][kk & 3], bi = sti[kk >> 2][kk & 3];
+        // g_{K-1}[k][c0 + jc] = g[piv[k]][pinv[c0 + jc]] (pivot order)
+        const cx<R> y = jc < nR ? g[(int)gjb.piv[4 * kk + lg] * LD + (int)gjb.pinv[c0 + jc]]
+                                : cx<R>{0, 0};
+        qr = mfma<R>::op(y.re, br, qr);
+        qr = mfma<R>::op(-y.im, bi, qr);
+        qi = mfma<R>::op(y.re, bi, qi);
+        qi = mfma<R>::op(y.im, br, qi);
+      }
+      const int i = 16 * wv + li;  // row of Q
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = 16 * J + lg + 4 * r;  // column of Q
+        if (i < nL && j < nR) S[i * SLD + j] = cx<R>{qr[r], qi[r]};
+      }
+    }
+  } else {
+    wg_gemm<R, NW, MAXT_SQ>(
+        nL, nR, nlast, [&](int i, int k) { return S[i * SLD + k]; },
+        [&](int k, int j) { return g[k * LD + c0 + j]; },
+        [&](int i, int j, cx<R> v) { S[i * SLD + j] = v; });
+  }
   __syncthreads();
   cx<R>* V = g;
   wg_gemm<R, NW, MAXT_SQ>(
@@ -296,7 +391,8 @@ static hipError_t launch_nmax(const RgfArgs& a, bool ortho, hipStream_t st) {
 // spills) measured 87.5 ms against 76.5 ms on the ribbon W = 48, K = 128 sweep.
 template <typename R, int NMAX>
 static hipError_t launch_sel(const RgfArgs& a, bool uniform, int nmax, bool ortho, hipStream_t st) {
-  if (uniform && nmax == NMAX) return launch_nmax<R, NMAX, true>(a, ortho, st);
+  if (uniform && nmax == NMAX)
+    return launch_nmax<R, NMAX, true, false, rgf_threads<R, NMAX, true, false>()>(a, ortho, st);
   return launch_nmax<R, NMAX, false>(a, ortho, st);
 }
 

@@ -336,8 +336,8 @@ template <typename R, int N>
 struct GJBScratch {
   cx<R> F[N][4];  // rank-4 multipliers of the current panel
   int pr[4];      // pivot rows of the current panel
-  int piv[N];     // piv[j]: row pivoting column j
-  int pinv[N];
+  short piv[N];   // piv[j]: row pivoting column j (16 bits: the 48-orbital
+  short pinv[N];  // transmission sweep fits four workgroups' LDS in one CU)
   int singular;
 };
 
@@ -610,6 +610,25 @@ __device__ __forceinline__ int wg_gj_blocked_core_tall(cx<R>* A, GJBScratch<R, N
     __syncthreads();
   }
   return sh->singular;
+}
+
+// The blocked Gauss-Jordan of an N x N matrix (N <= 64) left in pivot order:
+// A^-1[i][k] = A[piv[i]][pinv[k]] with piv / pinv in sh on return, so a caller
+// that reads through the two maps saves the reordering pass (and its
+// registers) of wg_gj_inverse_blocked.  Ends with a workgroup barrier; returns
+// nonzero if an exactly-zero pivot was met.
+template <typename R, int NT, int N, int LDA>
+__device__ __forceinline__ int wg_gj_inverse_pivoted(cx<R>* A, GJBScratch<R, N>* sh) {
+  const int t = threadIdx.x;
+  const int sing = wg_gj_blocked_core<R, NT, N, LDA>(A, sh);
+  for (int j = t; j < N; j += NT) sh->pinv[j] = j;
+  __syncthreads();
+  for (int j = t; j < N; j += NT) {
+    const int pj = sh->piv[j];
+    if (pj >= 0 && pj < N) sh->pinv[pj] = j;
+  }
+  __syncthreads();
+  return sing;
 }
 
 // In-place inverse of the N x N matrix A (LDS, leading dimension LDA) by the

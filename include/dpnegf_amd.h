@@ -66,9 +66,9 @@ int64_t dpnegf_arena_elems(int K, const int32_t* h_bsz);
  *                inverted in FP64 instead: DPNEGF_STATUS_REFINE_FALLBACK)
  *   T[p]       = Re Tr[Gamma_L G_{0,K-1} Gamma_R G_{0,K-1}^dagger]
  *   work       = device buffer of dpnegf_rgf_transmission_workspace_elems(K, h_bsz, P)
- *                complex elements (the sweep's carried G_{0,q} U_q of each point,
- *                kept in global memory by the kernels whose LDS would otherwise
- *                limit them to fewer workgroups per CU); contents on entry ignored
+ *                complex elements (per-point scratch: the corner of G_{0,K-1} of
+ *                the kernels that hold the sweep's carried G_{0,q} U_q outside
+ *                LDS); contents on entry ignored
  *   status[p]  = DPNEGF_STATUS_* bits (may be null)
  * Block sizes up to DPNEGF_TRANSMISSION_NMAX. */
 int dpnegf_rgf_transmission(void* stream, int precision, int K, const int32_t* h_bsz,
