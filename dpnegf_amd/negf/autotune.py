@@ -20,7 +20,7 @@ that fits in half the free memory when that is smaller.
 (m, n, k) and batched inverse n up in a process-wide table before it launches
 it (``dpnegf_tune_set``); keys without an entry keep the built-in choice (64-
 or 32-wide output tiles by how full the last tile is; the LDS blocked inverse
-for 65-96 orbitals, the register-resident one elsewhere).  ``tune_kernels``
+for 17-96 orbitals, the register-resident one elsewhere).  ``tune_kernels``
 runs the caller's path once with the library recording the keys it looks up,
 times every candidate for each key at the caller's batch size on random
 operands, and installs the fastest where it beats the built-in choice by more

@@ -305,7 +305,10 @@ def test_tuning_builtin_choices():
     assert lib.dpnegf_tune_builtin(nat.TUNE_GEMM_TILE, 96, 168, 96) == nat.TILE_32x64
     assert lib.dpnegf_tune_builtin(nat.TUNE_GEMM_TILE, 120, 144, 8) == nat.TILE_64x32
     assert lib.dpnegf_tune_builtin(nat.TUNE_GEMM_TILE, 480, 128, 64) == nat.TILE_64x64
-    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 64, 0, 0) == nat.INV_REGISTER
-    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 65, 0, 0) == nat.INV_LDS
+    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 16, 0, 0) == nat.INV_REGISTER
+    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 17, 0, 0) == nat.INV_LDS
+    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 64, 0, 0) == nat.INV_LDS
+    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 96, 0, 0) == nat.INV_LDS
+    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 97, 0, 0) == nat.INV_REGISTER
     assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 97, 0, 0) == nat.INV_REGISTER
     assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 129, 0, 0) == 0
