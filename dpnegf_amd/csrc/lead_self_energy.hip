@@ -186,6 +186,19 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
 
   int status = 0;
   int it = 0;
+  // The stop test of iteration it is decided after the first panel barrier of
+  // iteration it + 1 (below the iteration limit): the first panel's sweep
+  // starts while the votes are written, and a workgroup barrier is saved.  A
+  // stop there discards that panel; the iteration's results (eps, eps_s) are
+  // complete, and `sing` holds the singular flag of the solves that counted.
+  bool voted = false, converged = false;
+  int sing = 0;
+  auto stop_votes = [&]() -> bool {
+    double anyw = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) anyw += red[w];
+    return anyw == 0.0;
+  };
   while (true) {
     ++it;
     // ---- [X_a | X_b] = W \ [alpha | beta] ----
@@ -198,6 +211,8 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
       // after publishing a panel it starts on the next one while the other
       // waves apply the published one.
       // Panel q of W tile ts (q a compile-time constant: the row length).
+      // Returns false (uniformly) when the previous iteration's votes, read
+      // after the first panel's barrier, say stop.
       auto panel = [&](auto qc, int ts, bool swp, cx<R>(&v)[4]) __attribute__((always_inline)) {
         constexpr int q = decltype(qc)::value;
         constexpr int NC = 16 - 4 * q;  // panel + further columns of the tile
@@ -221,15 +236,20 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
           }
         }
         __syncthreads();
+        if (q == 0 && ts == 0 && voted && stop_votes()) return false;
         used = GJ::mark_used(used, buf, &L.gj);
         // W tiles left of the next panel, and the sweeper's own, need no update
         if (!swp && (kind != 0 || jt > ts)) GJ::update(xr, xi, buf, &L.gj);
+        return true;
       };
       for (int ts = 0; ts < NT16; ++ts) {
         const bool swp = kind == 0 && jt == ts;
         cx<R> v[4];
         if (swp) GJ::tile_to_rows(xr, xi, v, &L.XS[0][0]);
-        panel(std::integral_constant<int, 0>{}, ts, swp, v);
+        if (!panel(std::integral_constant<int, 0>{}, ts, swp, v)) {
+          converged = true;
+          break;
+        }
         panel(std::integral_constant<int, 1>{}, ts, swp, v);
         panel(std::integral_constant<int, 2>{}, ts, swp, v);
         panel(std::integral_constant<int, 3>{}, ts, swp, v);
@@ -242,10 +262,18 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
           GJ::sweep(j0, buf, used, &L.gj);
         }
         __syncthreads();
+        if (p == 0 && voted && stop_votes()) {
+          converged = true;
+          break;
+        }
         used = GJ::mark_used(used, buf, &L.gj);
         // W tiles wholly left of the next panel are no longer needed
         if (kind != 0 || 16 * jt + 15 >= j0 + 4) GJ::update(xr, xi, buf, &L.gj);
       }
+    }
+    if (converged) {  // at iteration it - 1
+      --it;
+      break;
     }
     // X (register row i is row pinv[i] of X) to B-operand fragments
     if (kind != 0) {
@@ -294,15 +322,14 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
       // offsets and the constants of W = zS - eps come from values the
       // compiler cannot hoist out of the loop: a dozen loop-invariant
       // addresses and constants per matrix would otherwise be spilled and
-      // reloaded one by one before each access.  The read-modify-writes load
-      // every element before storing any (the compiler cannot prove the rows
-      // distinct, and interleaved accesses would each wait for memory).
+      // reloaded one by one before each access.  Each read-modify-write loads
+      // its elements before the product it adds, so the loads are in flight
+      // during the matrix-core steps, and stores after it.
       int nq = n;
       double one = 1.0, zr = z.re, zi = z.im;
       asm volatile("" : "+s"(nq), "+v"(one), "+v"(zr), "+v"(zi));
       const int lo = g * nq + col;
       const bool cin = col < n;
-      gemm_acc(Ab, XS + jt * WAVE);
       cx<R> old[E];
 #pragma unroll
       for (int el = 0; el < E; ++el) {
@@ -310,19 +337,20 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
         const int off = lo + (row - g) * nq;
         old[el] = (row < n && cin) ? epss[off] : cx<R>{0, 0};
       }
+      gemm_acc(Ab, XS + jt * WAVE);
 #pragma unroll
       for (int el = 0; el < E; ++el) {
         const int row = GJ::row_of(el, g);
         const int off = lo + (row - g) * nq;
         if (row < n && cin) epss[off] = old[el] + cx<R>{xr[el], xi[el]};
       }
-      gemm_acc(Aa, XS + (NT16 + jt) * WAVE);
 #pragma unroll
       for (int el = 0; el < E; ++el) {
         const int row = GJ::row_of(el, g);
         const int off = lo + (row - g) * nq;
         old[el] = (row < n && cin) ? eps[off] : cx<R>{0, 0};
       }
+      gemm_acc(Aa, XS + (NT16 + jt) * WAVE);
 #pragma unroll
       for (int el = 0; el < E; ++el) {
         const int row = GJ::row_of(el, g);
@@ -343,6 +371,7 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
       gemm_acc(kind == 1 ? Aa : Ab, XS + ((kind - 1) * NT16 + jt) * WAVE);
     }
     __syncthreads();
+    sing = L.gj.singular;  // no solve writes it again before this iteration's vote is read
     if (kind != 0) {
       cx<R>* A = kind == 1 ? Aa : Ab;
 #pragma unroll
@@ -368,17 +397,14 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
     }
     go_on = __any(go_on);
     if (lane == 0) red[wave] = go_on ? 1.0 : 0.0;
-    __syncthreads();
-    double anyw = 0.0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) anyw += red[w];
-    if (anyw == 0.0) break;
-    if (it >= 101) {
-      status |= STATUS_LS_NOCONV;
+    if (it >= 101) {  // the limit: decide now (surface_green.py:222-224)
+      __syncthreads();
+      if (!stop_votes()) status |= STATUS_LS_NOCONV;
       break;
     }
+    voted = true;
   }
-  if (L.gj.singular) status |= STATUS_SINGULAR;
+  if (sing) status |= STATUS_SINGULAR;
   __syncthreads();
 
   // g_s = (zS - eps_s)^-1, written over eps_s (the g_s output)   (:209)

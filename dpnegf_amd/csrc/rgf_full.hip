@@ -50,23 +50,26 @@ __device__ __forceinline__ OpView op_view(const Opnd& o) {
 // through LDS in chunks of KC = 16: the 256 threads load the A chunk (TM x 16,
 // thread t at column k = t % 16 of rows t / 16 + 16 u) and the B chunk
 // (16 x TN, row t / 16, columns t % 16 + 16 u), so sixteen consecutive lanes
-// read sixteen consecutive elements.  Double buffered: the next chunk's raw h
-// (and s) values are loaded into registers before this chunk's matrix-core
-// steps and turned into operand values (h - z s, the conjugation, zeros outside
-// the matrix) only when they are stored to LDS after those steps, so no
-// arithmetic waits on the loads in between.  Indices are clamped, so every load
+// read sixteen consecutive elements.  The next chunk's raw h (and s) values
+// are loaded into registers before this chunk's matrix-core steps and turned
+// into operand values (h - z s, the conjugation, zeros outside the matrix)
+// only when they are stored to LDS after those steps, so no arithmetic waits
+// on the loads in between.  One LDS buffer (33 KB at 64 x 64) between two
+// barriers per chunk, and registers held to three waves per SIMD: three
+// workgroups share a CU (a second buffer, at two workgroups per CU, measured
+// 5-8% slower at 96-256).  Indices are clamped, so every load
 // is in bounds; a chunk past K contributes zeros.  A's LDS rows are padded to
 // 17 elements so that the fragment reads (16 rows at one k) spread over the
 // banks.  Matrix-core work is skipped, with wave-uniform control flow, for
 // 16 x 16 fragment tiles outside the output and for k-steps past K.
 template <bool SA, bool SB, int FR, int FC>
-__global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
+__global__ __launch_bounds__(256, 3) void bgemm_lds_kernel(BGemmArgs a) {
   static_assert((FR == 1 || FR == 2) && (FC == 1 || FC == 2), "fragments per wave");
   typedef mfma<double>::v4 v4;
   constexpr int TM = 32 * FR, TN = 32 * FC, UA = 2 * FR, UB = 2 * FC;
   constexpr int KC = 16, LDAS = KC + 1, LDBS = TN;
-  __shared__ cx<double> As[2][TM * LDAS];
-  __shared__ cx<double> Bs[2][KC * LDBS];
+  __shared__ cx<double> As[TM * LDAS];
+  __shared__ cx<double> Bs[KC * LDBS];
   const int tn = (a.n + TN - 1) / TN;
   const int ti = blockIdx.x / tn, tj = blockIdx.x % tn;
   const int b = blockIdx.y;
@@ -114,21 +117,21 @@ __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
       if (SB) rsb[u] = Bsp[boff[u] + kb];
     }
   };
-  auto store_chunk = [&](int k0, int buf) __attribute__((always_inline)) {
+  auto store_chunk = [&](int k0) __attribute__((always_inline)) {
     const bool kaok = k0 + ak < K, kbok = k0 + bk < K;
 #pragma unroll
     for (int u = 0; u < UA; ++u) {
       cx<double> va = ra[u];
       if (SA) va = va - zb * rsa[u];
       const bool oa = arok[u] && kaok;
-      As[buf][(ar + 16 * u) * LDAS + ak] = {oa ? va.re : 0.0, oa ? A.csign * va.im : 0.0};
+      As[(ar + 16 * u) * LDAS + ak] = {oa ? va.re : 0.0, oa ? A.csign * va.im : 0.0};
     }
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
       cx<double> vb = rb[u];
       if (SB) vb = vb - zb * rsb[u];
       const bool ob = bcok[u] && kbok;
-      Bs[buf][bk * LDBS + bc + 16 * u] = {ob ? vb.re : 0.0, ob ? Bv.csign * vb.im : 0.0};
+      Bs[bk * LDBS + bc + 16 * u] = {ob ? vb.re : 0.0, ob ? Bv.csign * vb.im : 0.0};
     }
   };
   v4 cr[FR][FC], ci[FR][FC];
@@ -137,10 +140,10 @@ __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
 #pragma unroll
     for (int y = 0; y < FC; ++y) cr[x][y] = ci[x][y] = v4{0, 0, 0, 0};
   const int rbase = 16 * FR * (w / 2), cbase = 16 * FC * (w % 2);
-  // NK matrix-core k-steps of the chunk in LDS buffer buf, for the RX x CY
+  // NK matrix-core k-steps of the chunk in LDS, for the RX x CY
   // fragment tiles of this wave that overlap the output (the others would only
   // multiply the zero padding).
-  auto ksteps = [&](auto RXc, auto CYc, auto NKc, int buf) __attribute__((always_inline)) {
+  auto ksteps = [&](auto RXc, auto CYc, auto NKc) __attribute__((always_inline)) {
     constexpr int RX = decltype(RXc)::value, CY = decltype(CYc)::value;
     constexpr int NK = decltype(NKc)::value;
     static_assert(RX <= FR && CY <= FC, "fragments beyond the wave's block");
@@ -149,9 +152,9 @@ __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
       const int kl = 4 * kk + lk;
       cx<double> pa[FR], pb[FC];
 #pragma unroll
-      for (int x = 0; x < RX; ++x) pa[x] = As[buf][(rbase + 16 * x + li) * LDAS + kl];
+      for (int x = 0; x < RX; ++x) pa[x] = As[(rbase + 16 * x + li) * LDAS + kl];
 #pragma unroll
-      for (int y = 0; y < CY; ++y) pb[y] = Bs[buf][kl * LDBS + cbase + 16 * y + li];
+      for (int y = 0; y < CY; ++y) pb[y] = Bs[kl * LDBS + cbase + 16 * y + li];
 #pragma unroll
       for (int x = 0; x < RX; ++x)
 #pragma unroll
@@ -166,27 +169,26 @@ __global__ __launch_bounds__(256) void bgemm_lds_kernel(BGemmArgs a) {
   // K = nfull KC + rem.  The loop body has no branches, so the accumulators
   // stay in one register class across iterations; its last iteration loads the
   // chunk after nfull - 1 (the remainder, or a chunk past K: clamped in bounds,
-  // zeroed on store) into the other buffer, which the remainder steps then read.
+  // zeroed on store), which the remainder steps then read.
   const int nfull = K / KC, rem = K % KC;
   auto run = [&](auto RXc, auto CYc) __attribute__((always_inline)) {
     for (int ch = 0; ch < nfull; ++ch) {
-      const int buf = ch & 1;
       load_chunk((ch + 1) * KC);
-      ksteps(RXc, CYc, std::integral_constant<int, KC / 4>(), buf);
-      store_chunk((ch + 1) * KC, buf ^ 1);
+      ksteps(RXc, CYc, std::integral_constant<int, KC / 4>());
+      __syncthreads();
+      store_chunk((ch + 1) * KC);
       __syncthreads();
     }
-    const int buf = nfull & 1;
     switch ((rem + 3) / 4) {
-      case 1: ksteps(RXc, CYc, std::integral_constant<int, 1>(), buf); break;
-      case 2: ksteps(RXc, CYc, std::integral_constant<int, 2>(), buf); break;
-      case 3: ksteps(RXc, CYc, std::integral_constant<int, 3>(), buf); break;
-      case 4: ksteps(RXc, CYc, std::integral_constant<int, 4>(), buf); break;
+      case 1: ksteps(RXc, CYc, std::integral_constant<int, 1>()); break;
+      case 2: ksteps(RXc, CYc, std::integral_constant<int, 2>()); break;
+      case 3: ksteps(RXc, CYc, std::integral_constant<int, 3>()); break;
+      case 4: ksteps(RXc, CYc, std::integral_constant<int, 4>()); break;
       default: break;
     }
   };
   load_chunk(0);
-  store_chunk(0, 0);
+  store_chunk(0);
   __syncthreads();
   // fragment tiles of this wave inside the output (at most FR x FC), wave-uniform
   const int wr = __builtin_amdgcn_readfirstlane(i0 + rbase);
