@@ -152,7 +152,8 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
   cx<R>* const Aa = &L.Aa[0][0];
   cx<R>* const Ab = &L.Ab[0][0];
   vE xr, xi;
-  // alpha0 = h10 - z s10, beta0 = h01 - z s01, eps = eps_s = H, W = zS - H  (:170-177)
+  // alpha0 = h10 - z s10, beta0 = h01 - z s01, eps = eps_s = H, W = zS - H  (:170-177);
+  // the workspace holds D = eps - eps_s (zero here) and the g_s output eps_s
 #pragma unroll
   for (int el = 0; el < E; ++el) {
     const int row = GJ::row_of(el, g);
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
       v = cx<R>{row == col ? R(1) : R(0), 0};
       if (in) {
         const cx<R> h = H[row * n + col];
-        eps[row * n + col] = h;
+        eps[row * n + col] = cx<R>{0, 0};
         epss[row * n + col] = h;
         v = zS(row, col) - h;
       }
@@ -275,6 +276,24 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
       --it;
       break;
     }
+    // The lane's element offsets (row * n + col of element el) and the
+    // constants of W = zS - eps are formed from a copy of n the compiler
+    // cannot see through, once per group of accesses: formed once, a dozen
+    // addresses per matrix stay live across the products, are spilled, and
+    // every reload's wait for scratch memory also waits for the loads and
+    // stores in flight.
+    struct TileIdx {
+      int n, g, col;
+    };
+    auto fresh_n = [&]() {
+      TileIdx t = {n, g, col};
+      asm volatile("" : "+s"(t.n), "+v"(t.g), "+v"(t.col));
+      return t;
+    };
+    auto tile_off = [&](const TileIdx& t, int el) { return GJ::row_of(el, t.g) * t.n + t.col; };
+    auto tile_in = [&](const TileIdx& t, int el) {
+      return GJ::row_of(el, t.g) < t.n && t.col < t.n;
+    };
     // X (register row i is row pinv[i] of X) to B-operand fragments
     if (kind != 0) {
       const int jc = (kind - 1) * NT16 + jt;
@@ -282,6 +301,32 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
       for (int el = 0; el < E; ++el) {
         const int k = L.gj.pinv[GJ::row_of(el, g)];
         L.XS[(k >> 2) * 2 * NT16 + jc][16 * (k & 3) + c] = cx<R>{xr[el], xi[el]};
+      }
+    }
+    // The products' accumulators start from the stored sums, loaded before
+    // the barrier, so the loads overlap the other waves' fragment writes:
+    //   beta owner, column tile j:  eps_s[:, j] += beta X_a[:, j], then the new
+    //                               beta[:, j] = beta X_b[:, j];
+    //   W owner:                    D[:, j] += alpha X_b[:, j] with
+    //                               D = eps - eps_s, so eps = eps_s + D gets
+    //                               beta X_a + alpha X_b;
+    //   alpha owner:                the new alpha[:, j] = alpha X_a[:, j].
+    // Per SIMD that is 432 matrix-core steps either way (three alpha owners,
+    // or a W and a beta owner), and no wave waits for memory mid-product.
+    // (the X tile has left the registers: they take the loaded sums).  The
+    // loads are unconditional (a load under a lane mask is waited for inside
+    // its branch): outside the n x n matrix a lane loads element 0, a finite
+    // value that stays in the padding of the accumulator tile (the padded
+    // rows of alpha, beta and columns of X are zero, and only in-range
+    // elements are stored or enter W).
+    if (kind != 1) {
+      const cx<R>* src = __builtin_amdgcn_readfirstlane(kind) == 0 ? eps : epss;
+      const TileIdx q = fresh_n();
+#pragma unroll
+      for (int el = 0; el < E; ++el) {
+        const cx<R> d = src[tile_in(q, el) ? tile_off(q, el) : 0];
+        xr[el] = d.re;
+        xi[el] = d.im;
       }
     }
     __syncthreads();
@@ -311,62 +356,26 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
         }
       }
     };
-#pragma unroll
-    for (int el = 0; el < E; ++el) {
-      xr[el] = 0;
-      xi[el] = 0;
-    }
     const cx<R>* XS = &L.XS[0][0];
     if (kind == 0) {
-      // beta X_a enters eps_s and eps; alpha X_b then enters eps.  Row
-      // offsets and the constants of W = zS - eps come from values the
-      // compiler cannot hoist out of the loop: a dozen loop-invariant
-      // addresses and constants per matrix would otherwise be spilled and
-      // reloaded one by one before each access.  Each read-modify-write loads
-      // its elements before the product it adds, so the loads are in flight
-      // during the matrix-core steps, and stores after it.
-      int nq = n;
-      double one = 1.0, zr = z.re, zi = z.im;
-      asm volatile("" : "+s"(nq), "+v"(one), "+v"(zr), "+v"(zi));
-      const int lo = g * nq + col;
-      const bool cin = col < n;
-      cx<R> old[E];
-#pragma unroll
-      for (int el = 0; el < E; ++el) {
-        const int row = GJ::row_of(el, g);
-        const int off = lo + (row - g) * nq;
-        old[el] = (row < n && cin) ? epss[off] : cx<R>{0, 0};
-      }
-      gemm_acc(Ab, XS + jt * WAVE);
-#pragma unroll
-      for (int el = 0; el < E; ++el) {
-        const int row = GJ::row_of(el, g);
-        const int off = lo + (row - g) * nq;
-        if (row < n && cin) epss[off] = old[el] + cx<R>{xr[el], xi[el]};
-      }
-#pragma unroll
-      for (int el = 0; el < E; ++el) {
-        const int row = GJ::row_of(el, g);
-        const int off = lo + (row - g) * nq;
-        old[el] = (row < n && cin) ? eps[off] : cx<R>{0, 0};
-      }
       gemm_acc(Aa, XS + (NT16 + jt) * WAVE);
+      const TileIdx q = fresh_n();
+#pragma unroll
+      for (int el = 0; el < E; ++el)
+        if (tile_in(q, el)) eps[tile_off(q, el)] = cx<R>{xr[el], xi[el]};
+    } else {
+      if (kind == 2) {
+        gemm_acc(Ab, XS + jt * WAVE);
+        const TileIdx q = fresh_n();
+#pragma unroll
+        for (int el = 0; el < E; ++el)
+          if (tile_in(q, el)) epss[tile_off(q, el)] = cx<R>{xr[el], xi[el]};
+      }
 #pragma unroll
       for (int el = 0; el < E; ++el) {
-        const int row = GJ::row_of(el, g);
-        const int off = lo + (row - g) * nq;
-        cx<R> w = {row == col ? one : R(0), 0};
-        if (row < n && cin) {
-          const cx<R> ne = old[el] + cx<R>{xr[el], xi[el]};
-          eps[off] = ne;
-          const cx<R> zq = {zr, zi};
-          const cx<R> zs = ORTHO ? (row == col ? zq : cx<R>{0, 0}) : zq * S[off];
-          w = zs - ne;
-        }
-        xr[el] = w.re;
-        xi[el] = w.im;
+        xr[el] = 0;
+        xi[el] = 0;
       }
-    } else {
       // alpha X_a (the new alpha) or beta X_b (the new beta)
       gemm_acc(kind == 1 ? Aa : Ab, XS + ((kind - 1) * NT16 + jt) * WAVE);
     }
@@ -376,6 +385,26 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
       cx<R>* A = kind == 1 ? Aa : Ab;
 #pragma unroll
       for (int el = 0; el < E; ++el) A[fslot(el)] = cx<R>{xr[el], xi[el]};
+    } else {
+      // W = zS - eps_s - D for the next solve; eps_s[:, j] was stored by the
+      // beta owner of tile j before the barrier (unconditional loads, as above)
+      const TileIdx q = fresh_n();
+      double one = 1.0, zr = z.re, zi = z.im;
+      asm volatile("" : "+v"(one), "+v"(zr), "+v"(zi));
+      cx<R> es[E];
+#pragma unroll
+      for (int el = 0; el < E; ++el) es[el] = epss[tile_in(q, el) ? tile_off(q, el) : 0];
+#pragma unroll
+      for (int el = 0; el < E; ++el) {
+        const int row = GJ::row_of(el, g);
+        const bool in = tile_in(q, el);
+        const int off = in ? tile_off(q, el) : 0;
+        const cx<R> zq = {zr, zi};
+        const cx<R> zs = ORTHO ? (row == col ? zq : cx<R>{0, 0}) : zq * S[off];
+        const cx<R> w = zs - (es[el] + cx<R>{xr[el], xi[el]});
+        xr[el] = in ? w.re : (row == col ? one : 0.0);
+        xi[el] = in ? w.im : 0.0;
+      }
     }
     if (threadIdx.x < N) L.gj.pinv[threadIdx.x] = threadIdx.x;
     __syncthreads();
