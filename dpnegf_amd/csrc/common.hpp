@@ -74,8 +74,10 @@ __device__ __forceinline__ T wave_max(T v) {
 // Maximum of a 32-bit unsigned value over a full wave (all 64 lanes active),
 // returned wave-uniform.  Four DPP steps leave every lane with the maximum of
 // its 16-lane row (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
-// row_mirror); the four row maxima are then combined through readlane.  No
-// LDS round trip, unlike __shfl_xor (ds_bpermute).
+// row_mirror); the four row maxima are then combined through readlane and
+// scalar maxima (left to itself the compiler combined them with a vector
+// max3 and a readfirstlane, on the pivot search's critical path).  No LDS
+// round trip, unlike __shfl_xor (ds_bpermute).
 template <int CTRL>
 __device__ __forceinline__ unsigned dpp_max_step(unsigned v) {
   const unsigned o = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
@@ -90,8 +92,10 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
   const unsigned b = (unsigned)__builtin_amdgcn_readlane((int)v, 16);
   const unsigned c = (unsigned)__builtin_amdgcn_readlane((int)v, 32);
   const unsigned d = (unsigned)__builtin_amdgcn_readlane((int)v, 48);
-  const unsigned ab = a > b ? a : b, cd = c > d ? c : d;
-  return ab > cd ? ab : cd;
+  unsigned m;
+  asm("s_max_u32 %0, %1, %2\n\ts_max_u32 %0, %0, %3\n\ts_max_u32 %0, %0, %4"
+      : "=&s"(m) : "s"(a), "s"(b), "s"(c), "s"(d) : "scc");
+  return m;
 }
 
 // Sum over the workgroup; `red` is LDS scratch of >= nwaves entries.  All

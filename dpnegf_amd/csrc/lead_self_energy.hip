@@ -29,11 +29,12 @@
 // B-operand fragment order; alpha and beta stay in LDS as A-operand fragments
 // from the previous iteration.  Then, with no data leaving the owners:
 //   alpha tile j owner:  alpha X_a (column tile j) -> its registers (next RHS)
-//   beta  tile j owner:  beta  X_b                 -> its registers
-//   W     tile j owner:  beta X_a -> eps_s += ;  + alpha X_b -> eps += ;
-//                        W = zS - eps for the next solve
-// eps lives in the workspace and eps_s in the g_s output, each read and
-// written by the same lanes every iteration.
+//   beta  tile j owner:  eps_s += beta X_a (stored), then beta X_b -> its
+//                        registers
+//   W     tile j owner:  D += alpha X_b (stored), with D = eps - eps_s; then
+//                        W = zS - eps_s - D for the next solve
+// so eps = eps_s + D receives beta X_a + alpha X_b.  D lives in the workspace
+// and eps_s in the g_s output.
 #include <type_traits>
 
 #include "rgf_kernels.hpp"
@@ -132,13 +133,16 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
   const int wave = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
   const int g = lane >> 4, c = lane & 15;
   // Column tile of the wave: kind 0 W, 1 alpha, 2 beta.  With 9 waves dealt
-  // round-robin over the 4 SIMDs, waves 0, 4, 8 share one SIMD: they take the
-  // alpha tiles, and each W tile (twice the product work) shares a SIMD with
-  // one beta tile.
+  // round-robin over the 4 SIMDs, waves 0, 4, 8 share one SIMD.  Products:
+  // W and alpha owners 144 matrix-core steps per iteration, beta owners 288.
+  // Elimination: alpha and beta owners update every panel, W tile j sweeps
+  // its own four panels and is updated only before them.  The SIMDs hold
+  // {W0, alpha0, alpha1}, {W1, beta0}, {W2, beta1}, {alpha2, beta2}: 432
+  // product steps each, and at most two panel updates at a time.
   int kind = wave / NT16, jt = wave % NT16;
   if (NT16 == 3) {
-    kind = (wave & 3) == 0 ? 1 : wave < 4 ? 0 : 2;
-    jt = kind == 1 ? wave >> 2 : kind == 0 ? wave - 1 : wave - 5;
+    kind = wave < 3 ? 0 : (wave == 3 || wave == 4 || wave == 8) ? 1 : 2;
+    jt = kind == 0 ? wave : kind == 2 ? wave - 5 : (wave == 4 ? 0 : wave == 8 ? 1 : 2);
   }
   const int col = 16 * jt + c;
   // A-operand fragment slot of element el (row 16 I + g + 4 r, column col):

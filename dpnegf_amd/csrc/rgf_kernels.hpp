@@ -71,7 +71,7 @@ struct LeadArgs {
   int P;
   const cx<double>* e;       // [P] E + E_ref (no eta)
   double eta;                // eta_lead
-  cx<double>* eps;           // workspace [P, n, n]
+  cx<double>* eps;           // workspace [P, n, n] (fused kernel: D = eps - eps_s)
   cx<double>* gs;            // out [P, n, n]
   cx<double>* sigma;         // out [P, nred, nred]
   int32_t* iters;            // [P] or null
@@ -81,7 +81,8 @@ struct LeadArgs {
 };
 
 hipError_t launch_lead_self_energy(const LeadArgs& a, bool ortho, hipStream_t st);
-// Workspace (complex elements) of the fused kernel: eps per point.
+// Workspace (complex elements) of the fused kernel: n x n per point (the
+// decimation's running sum D = eps - eps_s; eps_s accumulates in g_s).
 int64_t lead_self_energy_workspace(int n, int P);
 
 }  // namespace negf
