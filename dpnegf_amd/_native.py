@@ -10,7 +10,7 @@ import torch
 
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libdpnegf_amd.so")
 
-ABI_VERSION = 5  # DPNEGF_ABI_VERSION of include/dpnegf_amd.h this binding matches
+ABI_VERSION = 6  # DPNEGF_ABI_VERSION of include/dpnegf_amd.h this binding matches
 
 DPNEGF_FP64 = 0
 DPNEGF_MIXED = 1
@@ -26,7 +26,7 @@ STATUS_BAD_INDEX = 32
 TUNE_GEMM_TILE = 0
 TUNE_INVERSE = 1
 TILE_32x32, TILE_32x64, TILE_64x32, TILE_64x64 = 1, 2, 3, 4
-INV_REGISTER, INV_LDS = 1, 2
+INV_REGISTER, INV_LDS, INV_SCHUR = 1, 2, 3
 
 TRANSMISSION_NMAX = 64
 LEAD_NMAX = 48

@@ -321,7 +321,7 @@ int dpnegf_rgf_full(void* stream, int K, const int32_t* h_bsz, const void* arena
   a.s_bil = (const cx<double>*)s_bil; a.gbi = (cx<double>*)gbi; a.gbd = (cx<double>*)gbd;
   a.gbl = (cx<double>*)gbl; a.gbu = (cx<double>*)gbu;
   a.work = (cx<double>*)work; a.status = status;
-  a.inv_ws = nmax > negf::GJ_NMAX_LARGE ? a.work + 5 * (int64_t)B * nmax * nmax : nullptr;
+  a.inv_ws = negf::inverse_workspace(nmax, B) > 0 ? a.work + 5 * (int64_t)B * nmax * nmax : nullptr;
   hipError_t e = negf::rgf_full(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "rgf_full");
   return 0;

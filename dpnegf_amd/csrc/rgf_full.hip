@@ -682,8 +682,13 @@ static int builtin_tile(int m, int n) {
 // The blocked Gauss-Jordan in LDS (binv_lds_kernel, padded to 32, 48, 64 or 96)
 // from 17 orbitals on; measured on 4096 blocks: 0.15 / 0.097 ms at n = 20,
 // 0.87 / 0.25 ms at 48, 1.37 / 0.57 ms at 64, 4.33 / 2.58 ms at 96 (register /
-// LDS form), level at 72.
-static int builtin_inverse(int n) { return (n > 16 && n <= 96) ? INV_LDS : INV_REGISTER; }
+// LDS form), level at 72.  From 97 on the Schur-complement form: its halves
+// take the LDS form and its products run on the matrix cores, where the
+// 1024-thread register sweep reaches about 4 TF/s (the large-block sweep with
+// 120-orbital blocks: 617 -> 564 ms).
+static int builtin_inverse(int n) {
+  return n <= 16 ? INV_REGISTER : n <= 96 ? INV_LDS : INV_SCHUR;
+}
 
 int tune_builtin(int kind, int m, int n, int k) {
   if (kind == TUNE_GEMM_TILE && m >= 1 && n >= 1 && k >= 1) return builtin_tile(m, n);
@@ -696,8 +701,9 @@ int tune_set(int kind, int m, int n, int k, int choice) {
   if (kind == TUNE_GEMM_TILE) {
     if (m < 1 || n < 1 || k < 1 || choice < 0 || choice > TILE_64x64) return -1;
   } else if (kind == TUNE_INVERSE) {
-    if (n != 0 || k != 0 || m < 1 || m > GJ_NMAX_LARGE || choice < 0 || choice > INV_LDS) return -1;
+    if (n != 0 || k != 0 || m < 1 || m > GJ_NMAX_LARGE || choice < 0 || choice > INV_SCHUR) return -1;
     if (choice == INV_LDS && m > 96) return -1;
+    if (choice == INV_SCHUR && m < SCHUR_NMIN) return -1;
   } else {
     return -1;
   }
@@ -776,7 +782,19 @@ static hipError_t gemm(hipStream_t st, int B, int m, int n, int k, Opnd A, Opnd 
   } while (0)
 
 int64_t inverse_workspace(int n, int B) {
-  return n > GJ_NMAX_LARGE ? 3 * (int64_t)B * n * n : 0;
+  return n >= SCHUR_NMIN ? 3 * (int64_t)B * n * n : 0;
+}
+
+// The form inverse() takes for n: the tuning table's, else the built-in one;
+// the Schur form needs the caller's workspace (without one, n <= 128 falls
+// back to the register sweep).  Above GJ_NMAX_LARGE always the Schur form.
+static int inverse_form(int n, bool has_ws) {
+  if (n > GJ_NMAX_LARGE) return INV_SCHUR;
+  int form = tune_lookup(TUNE_INVERSE, n, 0, 0);
+  if (form == 0 || (form == INV_LDS && n > 96) || (form == INV_SCHUR && n < SCHUR_NMIN))
+    form = builtin_inverse(n);
+  if (form == INV_SCHUR && !has_ws) form = INV_REGISTER;
+  return form;
 }
 
 static hipError_t block_copy(hipStream_t st, cx<double>* D, int ldd, int64_t sd,
@@ -788,10 +806,11 @@ static hipError_t block_copy(hipStream_t st, cx<double>* D, int ldd, int64_t sd,
   return hipGetLastError();
 }
 
-// In-place inverse of B contiguous n x n matrices.  n <= GJ_NMAX_LARGE: one
-// workgroup per matrix, Gauss-Jordan with partial pivoting.  Larger n (up to
-// BLOCK_NMAX): two diagonal halves h1 = n/2, h2 = n - h1, each inverted by this
-// same function (recursively, until the halves are at most GJ_NMAX_LARGE),
+// In-place inverse of B contiguous n x n matrices (form: inverse_form).  The
+// register and LDS forms: one workgroup per matrix, Gauss-Jordan with partial
+// pivoting.  The Schur form (up to BLOCK_NMAX): two diagonal halves h1 = n/2,
+// h2 = n - h1, each inverted by this same function (recursively, until the
+// halves are at most GJ_NMAX_LARGE),
 // joined through the Schur complement S = A22 - A21 A11^-1 A12:
 //   A^-1 = [[X + X A12 Y A21 X, -X A12 Y], [-Y A21 X, Y]],  X = A11^-1, Y = S^-1.
 // No pivoting across the halves: every matrix inverted on this path is
@@ -801,19 +820,17 @@ static hipError_t block_copy(hipStream_t st, cx<double>* D, int ldd, int64_t sd,
 // one Newton step in FP64, X <- X + X (I - A X), which squares its relative
 // error.  A_copy: an existing copy of A (same layout) for that step's residual,
 // which saves copying A into the workspace.  ws: inverse_workspace(n, B) elements.
-// A half above GJ_NMAX_LARGE takes the region Xc below as its own workspace:
+// Halves up to GJ_NMAX_LARGE take a one-workgroup Gauss-Jordan form (a Schur
+// form nested inside this one measured 1.7e-10 against the 1e-10 parity bar
+// on 256/192/240 blocks).  A half above GJ_NMAX_LARGE takes the region Xc
+// below as its own workspace:
 // Xc is not used until the Newton step, and it holds B n^2 elements, at least
 // the 3 B ceil(n/2)^2 a half needs for n >= 7.
 static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* status,
                           cx<double>* ws, const cx<double>* A_copy = nullptr,
                           const int32_t* skip = nullptr) {
   const int64_t s = (int64_t)n * n;
-  // the tuning table's form for n, else the built-in one
-  int form = 0;
-  if (n <= GJ_NMAX_LARGE) {
-    form = tune_lookup(TUNE_INVERSE, n, 0, 0);
-    if (form == 0 || (form == INV_LDS && n > 96)) form = builtin_inverse(n);
-  }
+  const int form = inverse_form(n, ws != nullptr);
   if (form == INV_LDS) {
     if (n <= 32)
       hipLaunchKernelGGL((binv_lds_kernel<32, 128>), dim3(B), dim3(128), 0, st, A, n, s, status, skip);
@@ -830,7 +847,7 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
     hipLaunchKernelGGL((binv_kernel<64, 512, 4>), dim3(B), dim3(512), 0, st, A, n, s, status, skip);
   else if (n <= 96)  // registers capped at 128: two workgroups per CU
     hipLaunchKernelGGL((binv_kernel<96, 512, 4>), dim3(B), dim3(512), 0, st, A, n, s, status, skip);
-  else if (n <= GJ_NMAX_LARGE)
+  else if (form == INV_REGISTER)
     hipLaunchKernelGGL((binv_kernel<GJ_NMAX_LARGE, 1024>), dim3(B), dim3(1024), 0, st, A, n, s, status, skip);
   else if (n <= BLOCK_NMAX) {
     if (!ws) return hipErrorInvalidValue;
@@ -853,11 +870,11 @@ static hipError_t inverse(hipStream_t st, cx<double>* A, int n, int B, int32_t* 
     if (A_copy) A0 = A_copy;
     else CHK(block_copy(st, A0w, n, s, A, n, s, n, n, B, skip));
     CHK(block_copy(st, X, h1, s1, A, n, s, h1, h1, B, skip));
-    CHK(inverse(st, X, h1, B, status, Xc, nullptr, skip));
+    CHK(inverse(st, X, h1, B, status, h1 > GJ_NMAX_LARGE ? Xc : nullptr, nullptr, skip));
     CHK(gemm(st, B, h1, h2, h1, M(X, h1, s1), M(A12, n, s), one, zero, T1, s12, h2, nullptr, nullptr, skip));
     CHK(block_copy(st, Y, h2, s2, A22, n, s, h2, h2, B, skip));
     CHK(gemm(st, B, h2, h2, h1, M(A21, n, s), M(T1, h2, s12), mone, one, Y, s2, h2, nullptr, nullptr, skip));
-    CHK(inverse(st, Y, h2, B, status, Xc, nullptr, skip));
+    CHK(inverse(st, Y, h2, B, status, h2 > GJ_NMAX_LARGE ? Xc : nullptr, nullptr, skip));
     CHK(gemm(st, B, h2, h1, h1, M(A21, n, s), M(X, h1, s1), one, zero, T2, s12, h1, nullptr, nullptr, skip));
     CHK(gemm(st, B, h1, h2, h2, M(T1, h2, s12), M(Y, h2, s2), mone, zero, A12, s, n, nullptr, nullptr, skip));
     CHK(gemm(st, B, h2, h1, h2, M(Y, h2, s2), M(T2, h1, s12), mone, zero, A21, s, n, nullptr, nullptr, skip));
@@ -1282,9 +1299,9 @@ hipError_t lead_ls_batched(const LeadArgs& a, cx<double>* work, hipStream_t st) 
   hipLaunchKernelGGL(ls_zs_minus_kernel, dim3(gridn), dim3(256), 0, st, W, epss, a.lead,
                      a.lead_stride, sys_idx, a.e, n, B, a.eta, nullptr);
   CHK(hipGetLastError());
-  if (n <= GJ_NMAX_LARGE) {
+  if (inverse_form(n, true) != INV_SCHUR) {
     // one FP64 Newton step on the one-workgroup inverse, X <- X + X (I - A X)
-    // (above GJ_NMAX_LARGE the two-level inverse takes its own)
+    // (the Schur form takes its own)
     CHK(hipMemcpyAsync(X1, W, sizeof(cx<double>) * B * nn, hipMemcpyDeviceToDevice, st));
     CHK(inverse(st, W, n, B, status, iws));
     CHK(gemm(st, B, n, n, n, M(X1), M(W), mone, zero, X2, nn, n, nullptr));

@@ -7,10 +7,12 @@ namespace negf {
 enum { OP_N = 0, OP_H = 1, OP_T = 2, OP_C = 3 };
 // One-workgroup batched Gauss-Jordan inverse up to GJ_NMAX_LARGE (a 1024-thread
 // register-resident sweep; beyond 128 its column stripes spill), recursive
-// Schur-complement inverse over diagonal halves above that, up to BLOCK_NMAX.
+// Schur-complement inverse over diagonal halves above that, up to BLOCK_NMAX,
+// and by default from SCHUR_NMIN on wherever the caller gives a workspace.
 constexpr int GJ_NMAX_LARGE = 128;
 constexpr int BLOCK_NMAX = 1024;
-// Workspace of the batched inverse for n x n blocks (0 up to GJ_NMAX_LARGE).
+constexpr int SCHUR_NMIN = 97;
+// Workspace of the batched inverse for n x n blocks (0 below SCHUR_NMIN).
 int64_t inverse_workspace(int n, int B);
 // In-place inverse of B contiguous n x n blocks, 1 <= n <= GJ_NMAX_LARGE.
 hipError_t zinv_batched(hipStream_t st, cx<double>* A, int n, int B, int32_t* status);
@@ -19,12 +21,13 @@ hipError_t zinv_batched(hipStream_t st, cx<double>* A, int n, int B, int32_t* st
 // when they pick a kernel shape.  TUNE_GEMM_TILE keys (m, n, k), choices
 // TILE_32x32 .. TILE_64x64 (output tile rows x columns per workgroup);
 // TUNE_INVERSE keys (n, 0, 0), choices INV_REGISTER (register-resident
-// Gauss-Jordan, n <= 128) and INV_LDS (blocked Gauss-Jordan in LDS, n <= 96).
+// Gauss-Jordan, n <= 128), INV_LDS (blocked Gauss-Jordan in LDS, n <= 96) and
+// INV_SCHUR (Schur complement over two halves plus a Newton step, 97..128).
 // Choice 0 clears an entry.  With recording on, every (kind, m, n, k) a driver
 // looks up is collected (in first-seen order) for the autotuner to time.
 enum { TUNE_GEMM_TILE = 0, TUNE_INVERSE = 1 };
 enum { TILE_32x32 = 1, TILE_32x64 = 2, TILE_64x32 = 3, TILE_64x64 = 4 };
-enum { INV_REGISTER = 1, INV_LDS = 2 };
+enum { INV_REGISTER = 1, INV_LDS = 2, INV_SCHUR = 3 };
 int tune_set(int kind, int m, int n, int k, int choice);  // 0, or -1 for a bad argument
 int tune_get(int kind, int m, int n, int k);              // 0 when not set
 int tune_builtin(int kind, int m, int n, int k);          // the choice without an entry (0: bad key)
@@ -75,7 +78,7 @@ struct RgfFullArgs {
   const cx<double>* s_bil;     // bilinear G S G source or null
   cx<double>* gbi; cx<double>* gbd; cx<double>* gbl; cx<double>* gbu;
   cx<double>* work;            // 5 * B * nmax^2
-  cx<double>* inv_ws;          // inverse_workspace(nmax, B) (null when nmax <= GJ_NMAX_LARGE)
+  cx<double>* inv_ws;          // inverse_workspace(nmax, B) (null when that is 0)
   int32_t* status;             // [B] or null
 };
 

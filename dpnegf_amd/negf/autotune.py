@@ -140,6 +140,8 @@ def _candidates(key):
     kind, m, _, _ = key
     if kind == nat.TUNE_GEMM_TILE:
         return (nat.TILE_32x32, nat.TILE_32x64, nat.TILE_64x32, nat.TILE_64x64)
+    # the Schur form (97-128) is not timed: dpnegf_zinv_batched has no workspace
+    # for it, and its key keeps the built-in choice
     return (nat.INV_REGISTER, nat.INV_LDS) if m <= 96 else (nat.INV_REGISTER,)
 
 

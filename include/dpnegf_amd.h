@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define DPNEGF_ABI_VERSION 5
+#define DPNEGF_ABI_VERSION 6
 
 #define DPNEGF_OK 0
 #define DPNEGF_ERR_ARG (-1)
@@ -149,9 +149,14 @@ int64_t dpnegf_autotune_query(int K, const int32_t* h_bsz, int64_t mem_bytes);
  *    DPNEGF_TILE_32x32 / _32x64 / _64x32 / _64x64, the output tile (rows x
  *    columns) of one workgroup.
  *  - DPNEGF_TUNE_INVERSE, key (n, 0, 0) of a batched inverse: choice
- *    DPNEGF_INV_REGISTER (register-resident Gauss-Jordan, n <= 128) or
+ *    DPNEGF_INV_REGISTER (register-resident Gauss-Jordan, n <= 128),
  *    DPNEGF_INV_LDS (blocked Gauss-Jordan in LDS with matrix-core updates,
- *    n <= 96).  Inverses above 128 recurse on their halves, keyed by the half.
+ *    n <= 96) or DPNEGF_INV_SCHUR (inverses of the two diagonal halves joined
+ *    through the Schur complement on the matrix cores, then one FP64 Newton
+ *    step; 97 <= n <= 128, built in there).  Inverses above 128 always take
+ *    the Schur form and recurse on their halves, keyed by the half;
+ *    dpnegf_zinv_batched, which has no workspace, uses the register form
+ *    where the Schur form would be chosen.
  * Choice 0 removes an entry; keys without an entry keep the built-in choice.
  * Every choice gives the same result to rounding.  Set entries before a call
  * is enqueued or captured, not while another thread launches.  While
@@ -167,6 +172,7 @@ int64_t dpnegf_autotune_query(int K, const int32_t* h_bsz, int64_t mem_bytes);
 #define DPNEGF_TILE_64x64 4
 #define DPNEGF_INV_REGISTER 1
 #define DPNEGF_INV_LDS 2
+#define DPNEGF_INV_SCHUR 3
 int dpnegf_tune_set(int kind, int m, int n, int k, int choice);
 int dpnegf_tune_get(int kind, int m, int n, int k);
 /* The choice a key gets without a table entry; 0 for a key no driver can use. */

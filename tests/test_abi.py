@@ -283,6 +283,10 @@ def test_tuning_table_without_a_device():
         assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 90, 0, 0, nat.INV_LDS) == 0
         assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 97, 0, 0, nat.INV_LDS) != 0
         assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 129, 0, 0, nat.INV_REGISTER) != 0
+        assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 96, 0, 0, nat.INV_SCHUR) != 0
+        assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 100, 0, 0, nat.INV_SCHUR) == 0
+        assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 100, 0, 0, nat.INV_REGISTER) == 0
+        assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 100, 0, 0, 4) != 0
         assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 90, 1, 0, nat.INV_LDS) != 0
         assert lib.dpnegf_tune_set(2, 1, 1, 1, 1) != 0
         assert lib.dpnegf_tune_set(nat.TUNE_INVERSE, 90, 0, 0, 0) == 0
@@ -299,7 +303,7 @@ def test_tuning_table_without_a_device():
 def test_tuning_builtin_choices():
     """The built-in choice the autotuner compares against: 32-wide tiles in a
     dimension below 256 whose last 64-wide tile is at most half full, the LDS
-    inverse for 65..96 orbitals."""
+    inverse for 17..96 orbitals, the Schur-complement inverse for 97..128."""
     lib = nat.load()
     assert lib.dpnegf_tune_builtin(nat.TUNE_GEMM_TILE, 96, 96, 96) == nat.TILE_32x32
     assert lib.dpnegf_tune_builtin(nat.TUNE_GEMM_TILE, 96, 168, 96) == nat.TILE_32x64
@@ -309,6 +313,6 @@ def test_tuning_builtin_choices():
     assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 17, 0, 0) == nat.INV_LDS
     assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 64, 0, 0) == nat.INV_LDS
     assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 96, 0, 0) == nat.INV_LDS
-    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 97, 0, 0) == nat.INV_REGISTER
-    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 97, 0, 0) == nat.INV_REGISTER
+    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 97, 0, 0) == nat.INV_SCHUR
+    assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 128, 0, 0) == nat.INV_SCHUR
     assert lib.dpnegf_tune_builtin(nat.TUNE_INVERSE, 129, 0, 0) == 0

@@ -22,14 +22,17 @@ def _dev():
     return torch.device("cuda")
 
 
-def test_batched_lopez_sancho_n64_matches_oracle():
+@pytest.mark.parametrize("W", [64, 104])
+def test_batched_lopez_sancho_matches_oracle(W):
+    """Leads above 48 orbitals (batched decimation); at 104 the decimation's
+    inverse takes the Schur-complement form."""
     from dpnegf_amd.negf.surface_green import self_energy_batched
     from tests.test_gpu_rgf import _leads_sigmas_oracle, check_surface
     dev = _dev()
-    s = wl.ribbon(W=64, K=2, disorder=0.0)
+    s = wl.ribbon(W=W, K=2, disorder=0.0)
     E = np.array([-2.9, -0.7, 0.0, 1.3])
     eta = 1e-6
-    print("case ribbon64 leads", flush=True)
+    print(f"case ribbon{W} leads", flush=True)
     ref = _leads_sigmas_oracle(s, E, eta)
     for lead, r in zip((s.lead_L, s.lead_R), ref):
         res = self_energy_batched([tuple(x.to(dev) for x in lead)], _f64(E), eta,
@@ -56,10 +59,11 @@ def test_blocked_transmission_uneven_large_blocks():
     assert np.abs(T - np.array(ref)).max() < 1e-10 * max(1.0, np.abs(ref).max()), (T, ref)
 
 
-@pytest.mark.parametrize("sizes", [[256, 192, 240], [144, 168, 136]])
+@pytest.mark.parametrize("sizes", [[256, 192, 240], [144, 168, 136], [100, 128, 97]])
 def test_recursive_gf_blocks_up_to_256(sizes):
-    """Blocks above 128 (two-level inverse): every output block of
-    recursive_gf for uneven blocks with non-orthogonal S."""
+    """Blocks above 96 (the Schur-complement inverse: two halves and a
+    Newton step): every output block of recursive_gf for uneven blocks with
+    non-orthogonal S."""
     from dpnegf_amd.negf.recursive_green_cal import recursive_gf
     from tests.test_gpu_rgf import _cmp_tuple, _rand_se
     dev = _dev()
