@@ -24,7 +24,8 @@
 //
 // Layout: 3 NMAX/16 waves; wave w owns 16-column tile w of [W | alpha | beta]
 // in registers (the register-resident elimination of wg_gj_regs.hpp), so the
-// solve moves only panels through LDS.  After it the alpha / beta owners hold
+// solve moves only panels through LDS (at NMAX = 48 each panel is published
+// with a flag instead of a workgroup barrier; see the solve).  After it the alpha / beta owners hold
 // X = [X_a | X_b] (rows in pivot order), which goes to LDS in matrix-core
 // B-operand fragment order; alpha and beta stay in LDS as A-operand fragments
 // from the previous iteration.  Then, with no data leaving the owners:
@@ -47,6 +48,9 @@ namespace negf {
 template <int N>
 struct LsLoopShared {
   static constexpr int NT16 = N / 16, KS = N / 4;
+  int pflag[KS];  // panel p published in iteration pflag[p] & 0xffff; bit 16 on panel 0: stop
+  int nvote;      // stop-test votes written so far (NW per iteration)
+  int abort;      // a wait ran past its limit: the point is given up
   typename RegGJ<N>::Shared gj;
   cx<double> Aa[NT16 * KS][WAVE];      // alpha, A-operand fragments (row tile, k-step)
   cx<double> Ab[NT16 * KS][WAVE];      // beta
@@ -68,6 +72,10 @@ struct LsShared {
 };
 
 template <int NMAX> constexpr int ls_threads() { return 3 * (NMAX / 16) * WAVE; }
+// Polls (each followed by s_sleep 1, about 64 clocks) before a wait for a
+// published panel or for the stop votes gives up: about 17M clocks, some
+// thousand times the longest wait of a working sweep.
+constexpr int SPIN_LIMIT = 1 << 18;
 
 template <int NMAX, bool ORTHO>
 __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(LeadArgs a) {
@@ -84,6 +92,11 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
   constexpr int LDP = N + 1;
   constexpr int NTILE = (N / 16) * (N / 16);
   constexpr int MAXT1 = (NTILE + NW - 1) / NW;
+  // tile_to_rows scratch (16 N elements) in the X fragments' space, after the
+  // KS panel slots, 256-byte aligned
+  constexpr int T2R_OFF =
+      ((KS * (int)sizeof(typename GJ::Slot) + 255) / 256) * 256 / (int)sizeof(cx<R>);
+  static_assert(NT16 != 3 || T2R_OFF + 16 * N <= KS * 2 * NT16 * WAVE, "panel slots + scratch in XS");
   static_assert(NW <= 16, "reduction slots");
   __shared__ LsShared<N> sh;
   double* const red = sh.red;
@@ -186,7 +199,12 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
     xi[el] = v.im;
   }
   if (threadIdx.x < N) L.gj.pinv[threadIdx.x] = threadIdx.x;
-  if (threadIdx.x == 0) L.gj.singular = 0;
+  if (threadIdx.x < KS) L.pflag[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    L.gj.singular = 0;
+    L.nvote = 0;
+    L.abort = 0;
+  }
   __syncthreads();
 
   int status = 0;
@@ -212,68 +230,120 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
       // The sweeping wave of W tile ts copies its tile to one row per lane
       // (v = the first panel, xr / xi = the other twelve columns) and sweeps
       // its four panels there, applying each pivot step to its remaining
-      // columns itself.  It needs no matrix-core update of its own tile, and
-      // after publishing a panel it starts on the next one while the other
-      // waves apply the published one.
-      // Panel q of W tile ts (q a compile-time constant: the row length).
-      // Returns false (uniformly) when the previous iteration's votes, read
-      // after the first panel's barrier, say stop.
-      auto panel = [&](auto qc, int ts, bool swp, cx<R>(&v)[4]) __attribute__((always_inline)) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int NC = 16 - 4 * q;  // panel + further columns of the tile
-        const int j0 = 16 * ts + 4 * q, buf = (j0 >> 2) & 1;
-        if (swp) {
-          cx<R> row[NC];
-#pragma unroll
-          for (int m = 0; m < NC; ++m) {
-            const int src = m + 4 * q - 4;  // register pair of column 4 q + m (v for the first panel)
-            if (src < 0) row[m] = v[m < 4 ? m : 0];
-            else row[m] = cx<R>{xr[src], xi[src]};
+      // columns itself; it needs no matrix-core update of its own tile.  Each
+      // panel is published into its own slot (in the X fragments' space, free
+      // during the solve) with a flag, and the other waves apply the panels in
+      // order as their flags appear: no workgroup barrier per panel, so the
+      // sweeper runs its four panels back to back and the owner of the next
+      // W tile starts as soon as it has applied them.  Waits are bounded: past
+      // the limit a wave sets `abort` and the point is given up (status
+      // STATUS_SYNC_TIMEOUT) instead of the workgroup hanging.
+      typedef typename GJ::Slot Slot;
+      Slot* const slots = reinterpret_cast<Slot*>(&L.XS[0][0]);
+      cx<R>* const t2r = &L.XS[0][0] + T2R_OFF;
+      auto wait_for = [&](const int* f, int want) -> int {
+        int v = 0;
+        for (int n = 0;; ++n) {
+          v = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if ((v & 0xffff) == want) break;
+          if (__hip_atomic_load(&L.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) ||
+              n >= SPIN_LIMIT) {
+            if (lane == 0) __hip_atomic_store(&L.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            v = 1 << 16;  // stop
+            break;
           }
-          GJ::template sweep_row<NC>(row, j0, buf, used, &L.gj);
-#pragma unroll
-          for (int m = 0; m < NC; ++m) {
-            const int dst = m + 4 * q - 4;
-            if (dst >= 0) {
-              xr[dst] = row[m].re;
-              xi[dst] = row[m].im;
-            }
-          }
+          __builtin_amdgcn_s_sleep(1);
         }
-        __syncthreads();
-        if (q == 0 && ts == 0 && voted && stop_votes()) return false;
-        used = GJ::mark_used(used, buf, &L.gj);
-        // W tiles left of the next panel, and the sweeper's own, need no update
-        if (!swp && (kind != 0 || jt > ts)) GJ::update(xr, xi, buf, &L.gj);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        return v;
+      };
+      bool stop = false;
+      // Applies published panels [p0, p1) to the wave's tile; false on a stop.
+      auto apply_panels = [&](int p0, int p1) -> bool {
+        for (int pn = p0; pn < p1; ++pn) {
+          const int f = wait_for(&L.pflag[pn], it);
+          if (f >> 16) return false;
+          if (kind == 0) used = GJ::mark_used(used, &slots[pn]);
+          GJ::update(xr, xi, &slots[pn]);
+        }
         return true;
       };
-      for (int ts = 0; ts < NT16; ++ts) {
-        const bool swp = kind == 0 && jt == ts;
-        cx<R> v[4];
-        if (swp) GJ::tile_to_rows(xr, xi, v, &L.XS[0][0]);
-        if (!panel(std::integral_constant<int, 0>{}, ts, swp, v)) {
-          converged = true;
-          break;
+      if (kind == 0) {
+        // W owner of tile jt: the panels of the tiles left of it, then its own
+        if (!apply_panels(0, 4 * jt)) {
+          stop = true;
+        } else {
+          cx<R> v[4];
+          GJ::tile_to_rows(xr, xi, v, t2r);
+          // Panel q of the tile (q a compile-time constant: the row length).
+          auto sweep_panel = [&](auto qc) __attribute__((always_inline)) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int NC = 16 - 4 * q;  // panel + further columns of the tile
+            const int pn = 4 * jt + q;
+            cx<R> row[NC];
+#pragma unroll
+            for (int m = 0; m < NC; ++m) {
+              const int src = m + 4 * q - 4;  // register pair of column 4 q + m (v for the first panel)
+              if (src < 0) row[m] = v[m < 4 ? m : 0];
+              else row[m] = cx<R>{xr[src], xi[src]};
+            }
+            GJ::template sweep_row<NC>(row, 4 * pn, &slots[pn], used, &L.gj);
+#pragma unroll
+            for (int m = 0; m < NC; ++m) {
+              const int dst = m + 4 * q - 4;
+              if (dst >= 0) {
+                xr[dst] = row[m].re;
+                xi[dst] = row[m].im;
+              }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            used = GJ::mark_used(used, &slots[pn]);
+            // the first panel carries the previous iteration's stop decision
+            if (pn == 0 && it > 1) {
+              wait_for(&L.nvote, (NW * (it - 1)) & 0xffff);
+              stop = stop_votes() || L.abort;
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0)
+              __hip_atomic_store(&L.pflag[pn], it | (stop ? 1 << 16 : 0), __ATOMIC_RELEASE,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          };
+          sweep_panel(std::integral_constant<int, 0>{});
+          if (!stop) {
+            sweep_panel(std::integral_constant<int, 1>{});
+            sweep_panel(std::integral_constant<int, 2>{});
+            sweep_panel(std::integral_constant<int, 3>{});
+          }
         }
-        panel(std::integral_constant<int, 1>{}, ts, swp, v);
-        panel(std::integral_constant<int, 2>{}, ts, swp, v);
-        panel(std::integral_constant<int, 3>{}, ts, swp, v);
+      } else {
+        stop = !apply_panels(0, KS);
       }
+      // every panel applied (and the slots free) before X takes their space;
+      // the stop decision and an abort are uniform from here on
+      __syncthreads();
+      if (L.abort) {
+        status |= STATUS_SYNC_TIMEOUT;
+        --it;
+        break;
+      }
+      converged = stop;
     } else {
       for (int p = 0; p < KS; ++p) {
         const int j0 = 4 * p, buf = p & 1;
         if (kind == 0 && jt == (j0 >> 4)) {
           GJ::put_panel(xr, xi, j0 & 15, &L.gj);
-          GJ::sweep(j0, buf, used, &L.gj);
+          GJ::sweep(j0, &L.gj.slot[buf], used, &L.gj);
         }
         __syncthreads();
         if (p == 0 && voted && stop_votes()) {
           converged = true;
           break;
         }
-        used = GJ::mark_used(used, buf, &L.gj);
+        used = GJ::mark_used(used, &L.gj.slot[buf]);
         // W tiles wholly left of the next panel are no longer needed
-        if (kind != 0 || 16 * jt + 15 >= j0 + 4) GJ::update(xr, xi, buf, &L.gj);
+        if (kind != 0 || 16 * jt + 15 >= j0 + 4) GJ::update(xr, xi, &L.gj.slot[buf]);
       }
     }
     if (converged) {  // at iteration it - 1
@@ -430,6 +500,8 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
     }
     go_on = __any(go_on);
     if (lane == 0) red[wave] = go_on ? 1.0 : 0.0;
+    if (NT16 == 3 && lane == 0)
+      __hip_atomic_fetch_add(&L.nvote, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (it >= 101) {  // the limit: decide now (surface_green.py:222-224)
       __syncthreads();
       if (!stop_votes()) status |= STATUS_LS_NOCONV;

@@ -11,6 +11,7 @@ enum : int32_t {
   STATUS_LS_RESID_WARN = 8,  // Lopez-Sancho: Dyson residual > 1e-8
   STATUS_REFINE_FALLBACK = 16,  // mixed precision: refinement did not converge
   STATUS_BAD_INDEX = 32,     // sys_idx[p] outside [0, nsys): point skipped
+  STATUS_SYNC_TIMEOUT = 64,  // lead kernel: a wave waited past its limit for a panel; point given up
 };
 
 // Largest per-thread row count of wg_gj_inverse over all n <= nmax.

@@ -34,12 +34,16 @@ struct RegGJ {
   typedef double vE __attribute__((ext_vector_type(E)));
   typedef double v4 __attribute__((ext_vector_type(4)));
 
+  // What one panel publishes.
+  struct Slot {
+    cx<double> FM[NT16][WAVE];  // -Fm, A-operand fragment order
+    cx<double> L[4][4];
+    cx<double> dinv[4];
+    int pr[4];
+  };
   struct Shared {
-    cx<double> FM[2][NT16][WAVE];  // -Fm, A-operand fragment order, by panel parity
+    Slot slot[2];                  // by panel parity
     cx<double> PAN[N][4];          // the panel's columns, one row per entry
-    cx<double> L[2][4][4];
-    cx<double> dinv[2][4];
-    int pr[2][4];
     int piv[N];                    // piv[j]: row that pivots column j
     int pinv[N];                   // pinv[piv[j]] = j
     int singular;
@@ -81,12 +85,12 @@ struct RegGJ {
 
   // Sweeper, step 2: the four pivot steps of panel j0 on PAN, one row per
   // lane.  `used`: this lane's row has pivoted in an earlier panel.
-  static __device__ __forceinline__ void sweep(int j0, int buf, bool used, Shared* sh) {
+  static __device__ __forceinline__ void sweep(int j0, Slot* sl, bool used, Shared* sh) {
     const int lane = threadIdx.x & (WAVE - 1);
     cx<double> v[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) v[c] = lane < N ? sh->PAN[lane][c] : cx<double>{0, 0};
-    sweep_row<4>(v, j0, buf, used, sh);
+    sweep_row<4>(v, j0, sl, used, sh);
   }
 
   // The four pivot steps of panel j0 on a row held by this lane (lane = row):
@@ -106,16 +110,16 @@ struct RegGJ {
   // earlier steps, never scaled: they were not pivots yet), the reciprocals,
   // piv / pinv and the singular flag.
   template <int NC>
-  static __device__ __forceinline__ void sweep_row(cx<double> (&v)[NC], int j0, int buf,
+  static __device__ __forceinline__ void sweep_row(cx<double> (&v)[NC], int j0, Slot* sl,
                                                    bool used, Shared* sh) {
-    sweep_row<NC>([&](int c) { return v[c]; }, [&](int c, cx<double> x) { v[c] = x; }, j0, buf,
+    sweep_row<NC>([&](int c) { return v[c]; }, [&](int c, cx<double> x) { v[c] = x; }, j0, sl,
                   used, sh);
   }
   // Same, with column c of the row read and written through get(c) /
   // set(c, x) (c a compile-time constant after unrolling), so that a row held
   // in the tile's own registers is swept in place.
   template <int NC, class GET, class SET>
-  static __device__ __forceinline__ void sweep_row(GET get, SET set, int j0, int buf, bool used,
+  static __device__ __forceinline__ void sweep_row(GET get, SET set, int j0, Slot* sl, bool used,
                                                    Shared* sh) {
     const int lane = threadIdx.x & (WAVE - 1);
     const bool row_ok = lane < N;
@@ -139,7 +143,7 @@ struct RegGJ {
         used = true;
         pstep = jj;
 #pragma unroll
-        for (int k2 = 0; k2 < jj; ++k2) sh->L[buf][jj][k2] = F[k2];
+        for (int k2 = 0; k2 < jj; ++k2) sl->L[jj][k2] = F[k2];
       }
       F[jj] = get(jj);
       const cx<double> d = bcast_lane(F[jj], p);
@@ -168,14 +172,14 @@ struct RegGJ {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const cx<double> fm = (pstep < 4 && k <= pstep) ? cx<double>{0, 0} : F[k] * s;
-        sh->FM[buf][lane >> 4][(lane & 15) + 16 * k] = -fm;
+        sl->FM[lane >> 4][(lane & 15) + 16 * k] = -fm;
       }
     }
     if (lane == 0) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        sh->pr[buf][k] = pr[k];
-        sh->dinv[buf][k] = dinv[k];
+        sl->pr[k] = pr[k];
+        sl->dinv[k] = dinv[k];
         sh->piv[j0 + k] = pr[k];
         sh->pinv[pr[k]] = j0 + k;
       }
@@ -209,21 +213,21 @@ struct RegGJ {
 
   // After the panel's barrier, every wave: marks the panel's pivot rows as
   // used (lane = row).
-  static __device__ __forceinline__ bool mark_used(bool used, int buf, const Shared* sh) {
+  static __device__ __forceinline__ bool mark_used(bool used, const Slot* sl) {
     const int lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) used |= (lane == sh->pr[buf][k]);
+    for (int k = 0; k < 4; ++k) used |= (lane == sl->pr[k]);
     return used;
   }
 
   // After the panel's barrier, an owner: elimination step on its tile.
-  static __device__ __forceinline__ void update(vE& xr, vE& xi, int buf, const Shared* sh) {
+  static __device__ __forceinline__ void update(vE& xr, vE& xi, const Slot* sl) {
     const int lane = threadIdx.x & (WAVE - 1), g = lane >> 4, c = lane & 15;
     int pe[4], pg[4];
     cx<double> z[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int p = __builtin_amdgcn_readfirstlane(sh->pr[buf][k]);
+      const int p = __builtin_amdgcn_readfirstlane(sl->pr[k]);
       pe[k] = 4 * (p >> 4) + ((p & 15) >> 2);
       pg[k] = p & 3;
       const double zr = xr[pe[k]], zi = xi[pe[k]];
@@ -235,8 +239,8 @@ struct RegGJ {
     for (int k = 0; k < 4; ++k) {
       cx<double> acc = z[k];
 #pragma unroll
-      for (int k2 = 0; k2 < k; ++k2) acc = acc - sh->L[buf][k][k2] * u[k2];
-      u[k] = acc * sh->dinv[buf][k];
+      for (int k2 = 0; k2 < k; ++k2) acc = acc - sl->L[k][k2] * u[k2];
+      u[k] = acc * sl->dinv[k];
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -253,7 +257,7 @@ struct RegGJ {
     const double b23r = g1 ? u[3].re : u[2].re, b23i = g1 ? u[3].im : u[2].im;
     const cx<double> b = {g2 ? b23r : b01r, g2 ? b23i : b01i};
 #pragma unroll
-    for (int I = 0; I < NT16; ++I) cmfma(xr, xi, I, sh->FM[buf][I][lane], b);
+    for (int I = 0; I < NT16; ++I) cmfma(xr, xi, I, sl->FM[I][lane], b);
   }
 };
 

@@ -45,6 +45,8 @@ extern "C" {
 #define DPNEGF_STATUS_LS_RESID_WARN 8
 #define DPNEGF_STATUS_REFINE_FALLBACK 16
 #define DPNEGF_STATUS_BAD_INDEX 32 /* sys_idx[p] outside [0, nsys): point skipped */
+#define DPNEGF_STATUS_SYNC_TIMEOUT 64 /* internal: a lead-kernel wave waited past its
+                                        limit for a published panel; point given up */
 
 int dpnegf_abi_version(void);
 const char* dpnegf_last_error(void);
