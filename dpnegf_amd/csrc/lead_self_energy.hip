@@ -92,11 +92,12 @@ __global__ __launch_bounds__(ls_threads<NMAX>()) void lead_self_energy_kernel(Le
   constexpr int LDP = N + 1;
   constexpr int NTILE = (N / 16) * (N / 16);
   constexpr int MAXT1 = (NTILE + NW - 1) / NW;
-  // tile_to_rows scratch (16 N elements) in the X fragments' space, after the
+  // tile_to_rows scratch (17 N elements) in the X fragments' space, after the
   // KS panel slots, 256-byte aligned
   constexpr int T2R_OFF =
       ((KS * (int)sizeof(typename GJ::Slot) + 255) / 256) * 256 / (int)sizeof(cx<R>);
-  static_assert(NT16 != 3 || T2R_OFF + 16 * N <= KS * 2 * NT16 * WAVE, "panel slots + scratch in XS");
+  static_assert(NT16 != 3 || T2R_OFF + GJ::TB_LD * N <= KS * 2 * NT16 * WAVE,
+                "panel slots + scratch in XS");
   static_assert(NW <= 16, "reduction slots");
   __shared__ LsShared<N> sh;
   double* const red = sh.red;

@@ -188,26 +188,33 @@ struct RegGJ {
   }
 
   // The calling wave's tile (16 columns) to one row per lane through the
-  // 16 N-entry LDS buffer tb: v[0..3] = columns 0..3 of row `lane`, the
-  // other twelve columns into xr / xi (E >= 12), as (xr[m], xi[m]) = column
-  // 4 + m.  Lanes >= N get zeros.
+  // 17 N-entry LDS buffer tb (rows padded to 17 elements: with 16, every lane
+  // of the row reads hit the same four banks and took 2.1-2.6k cycles a
+  // tile): v[0..3] = columns 0..3 of row `lane`, the other twelve columns
+  // into xr / xi (E >= 12), as (xr[m], xi[m]) = column 4 + m.  Lanes >= N
+  // get zeros (they read row N - 1 and drop it: no load under a lane mask).
+  static constexpr int TB_LD = 17;
   static __device__ __forceinline__ void tile_to_rows(vE& xr, vE& xi, cx<double> (&v)[4],
                                                       cx<double>* tb) {
     static_assert(E >= 12, "row-per-lane tile needs twelve register pairs");
     const int lane = threadIdx.x & (WAVE - 1), g = lane >> 4, c = lane & 15;
 #pragma unroll
-    for (int e = 0; e < E; ++e) tb[row_of(e, g) * 16 + c] = cx<double>{xr[e], xi[e]};
+    for (int e = 0; e < E; ++e) tb[row_of(e, g) * TB_LD + c] = cx<double>{xr[e], xi[e]};
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const bool ok = lane < N;
+    const cx<double>* rowp = tb + (ok ? lane : N - 1) * TB_LD;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = ok ? tb[lane * 16 + k] : cx<double>{0, 0};
+    for (int k = 0; k < 4; ++k) {
+      const cx<double> t = rowp[k];
+      v[k] = cx<double>{ok ? t.re : 0.0, ok ? t.im : 0.0};
+    }
 #pragma unroll
     for (int m = 0; m < 12; ++m) {
-      const cx<double> t = ok ? tb[lane * 16 + 4 + m] : cx<double>{0, 0};
-      xr[m] = t.re;
-      xi[m] = t.im;
+      const cx<double> t = rowp[4 + m];
+      xr[m] = ok ? t.re : 0.0;
+      xi[m] = ok ? t.im : 0.0;
     }
   }
 
