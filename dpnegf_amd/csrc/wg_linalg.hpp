@@ -183,8 +183,9 @@ __device__ __forceinline__ cx<float> recip_nr(cx<float> d) {
 // always selectable (the selection below never returns an out-of-range row).
 template <typename R>
 __device__ __forceinline__ R pivot_key(cx<R> a) {
-  const R m = abs1(a);
-  return (m == m) ? m : R(0);
+  // max(m, 0) is m, or 0 for a NaN m (maxNum): one instruction on the chain
+  // instead of a compare and two selects
+  return __builtin_fmax(abs1(a), R(0));
 }
 
 // In-place style inverse: reads the matrix through load(i,k), writes the
