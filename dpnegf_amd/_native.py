@@ -10,7 +10,7 @@ import torch
 
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libdpnegf_amd.so")
 
-ABI_VERSION = 6  # DPNEGF_ABI_VERSION of include/dpnegf_amd.h this binding matches
+ABI_VERSION = 7  # DPNEGF_ABI_VERSION of include/dpnegf_amd.h this binding matches
 
 DPNEGF_FP64 = 0
 DPNEGF_MIXED = 1
@@ -57,10 +57,11 @@ _SIGS = {
                         _i32),
     "dpnegf_rgf_grad_workspace_elems": ([_i32, _vp, _i32], _i64),
     "dpnegf_rgf_transmission_grad": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _i32,
-                                      _vp, _vp, _vp, _vp, _vp, _vp], _i32),
+                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i32),
     "dpnegf_rgf_grad_ckpt_workspace_elems": ([_i32, _vp, _i32, _i32], _i64),
     "dpnegf_rgf_transmission_grad_ckpt": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32,
-                                           _i32, _vp, _vp, _vp, _vp, _i32, _vp, _vp], _i32),
+                                           _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp],
+                                          _i32),
     "dpnegf_batch_sum_conjt": ([_vp, _vp, _vp, _i32, _i32, _i32, _i32], _i32),
     "dpnegf_weighted_re_sum": ([_vp, _vp, _vp, _vp, _i64, _i32, _i32], _i32),
     "dpnegf_zgemm_batched": ([_vp, _i32, _i32, _i32, _i32, _vp, _i32, _i64, _i32, _vp, _i32, _i64,

@@ -336,8 +336,8 @@ int64_t dpnegf_rgf_grad_workspace_elems(int K, const int32_t* h_bsz, int B) {
 // < 0 on an argument error, 0 otherwise.
 static int grad_args(negf::RgfGradArgs& a, int K, const int32_t* h_bsz, const void* arena, int B,
                      const void* z, const void* sigL, int nL, const void* sigR, int nR, int ortho,
-                     const double* w, void* grad_hd, void* grad_hu, void* grad_hl, void* work,
-                     int32_t* status) {
+                     const double* w, void* grad_hd, void* grad_hu, void* grad_hl,
+                     void* grad_sigL, void* grad_sigR, void* work, int32_t* status) {
   int nmax = 0;
   if (int r = check_blocks(K, h_bsz, DPNEGF_FULL_NMAX, &nmax)) return r;
   if (B < 0) return fail(DPNEGF_ERR_ARG, "B must be >= 0");
@@ -351,7 +351,9 @@ static int grad_args(negf::RgfGradArgs& a, int K, const int32_t* h_bsz, const vo
   a.K = K; a.B = B; a.ortho = ortho; a.bsz = h_bsz; a.arena = (const cx<double>*)arena;
   a.z = (const cx<double>*)z; a.sigL = (const cx<double>*)sigL; a.nL = nL;
   a.sigR = (const cx<double>*)sigR; a.nR = nR; a.w = w; a.grad_hd = (cx<double>*)grad_hd;
-  a.grad_hu = (cx<double>*)grad_hu; a.grad_hl = (cx<double>*)grad_hl; a.work = (cx<double>*)work;
+  a.grad_hu = (cx<double>*)grad_hu; a.grad_hl = (cx<double>*)grad_hl;
+  a.grad_sigL = (cx<double>*)grad_sigL; a.grad_sigR = (cx<double>*)grad_sigR;
+  a.work = (cx<double>*)work;
   a.status = status;
   return 0;
 }
@@ -359,10 +361,11 @@ static int grad_args(negf::RgfGradArgs& a, int K, const int32_t* h_bsz, const vo
 int dpnegf_rgf_transmission_grad(void* stream, int K, const int32_t* h_bsz, const void* arena,
                                  int B, const void* z, const void* sigL, int nL, const void* sigR,
                                  int nR, int ortho, const double* w, void* grad_hd, void* grad_hu,
-                                 void* grad_hl, void* work, int32_t* status) {
+                                 void* grad_hl, void* grad_sigL, void* grad_sigR, void* work,
+                                 int32_t* status) {
   negf::RgfGradArgs a;
   const int r = grad_args(a, K, h_bsz, arena, B, z, sigL, nL, sigR, nR, ortho, w, grad_hd, grad_hu,
-                          grad_hl, work, status);
+                          grad_hl, grad_sigL, grad_sigR, work, status);
   if (r) return r < 0 ? r : 0;
   hipError_t e = negf::rgf_grad(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "rgf_transmission_grad");
@@ -377,12 +380,13 @@ int64_t dpnegf_rgf_grad_ckpt_workspace_elems(int K, const int32_t* h_bsz, int B,
 int dpnegf_rgf_transmission_grad_ckpt(void* stream, int K, const int32_t* h_bsz,
                                       const void* arena, int B, const void* z, const void* sigL,
                                       int nL, const void* sigR, int nR, int ortho, const double* w,
-                                      void* grad_hd, void* grad_hu, void* grad_hl, int stride,
-                                      void* work, int32_t* status) {
+                                      void* grad_hd, void* grad_hu, void* grad_hl,
+                                      void* grad_sigL, void* grad_sigR, int stride, void* work,
+                                      int32_t* status) {
   if (stride < 1) return fail(DPNEGF_ERR_ARG, "checkpoint stride %d < 1", stride);
   negf::RgfGradArgs a;
   const int r = grad_args(a, K, h_bsz, arena, B, z, sigL, nL, sigR, nR, ortho, w, grad_hd, grad_hu,
-                          grad_hl, work, status);
+                          grad_hl, grad_sigL, grad_sigR, work, status);
   if (r) return r < 0 ? r : 0;
   hipError_t e = negf::rgf_grad_ckpt(a, stride, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "rgf_transmission_grad_ckpt");

@@ -385,6 +385,25 @@ __global__ void bgamma_pad_kernel(cx<double>* D, const cx<double>* sig, int ns, 
   }
 }
 
+// Self-energy gradients, per energy: out[b][i][j] (+)= conj(a_b X[b][off + j][off + i])
+// for i, j < m, X a batch of ld x ld blocks, a_b = c (2 w[b] when w is given).
+__global__ void bcorner_conjt_kernel(cx<double>* out, const cx<double>* X, int ld, int off, int m,
+                                     int B, cx<double> c, const double* w, int accumulate) {
+  const int64_t tot = (int64_t)B * m * m;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < tot;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(x / ((int64_t)m * m));
+    const int e = (int)(x % ((int64_t)m * m));
+    const int i = e / m, j = e % m;
+    const cx<double> v = X[(int64_t)b * ld * ld + (int64_t)(off + j) * ld + off + i];
+    const double f = w ? 2.0 * w[b] : 1.0;
+    const cx<double> a = {c.re * f, c.im * f};
+    const cx<double> t = a * v;
+    const cx<double> r = {t.re, -t.im};
+    out[x] = accumulate ? out[x] + r : r;
+  }
+}
+
 // out[e] (+)= sum_b Re(c_b * in[b][e]) over a batch of B matrices of `count` elements
 __global__ void bweighted_re_sum_kernel(double* out, const cx<double>* in, const cx<double>* c,
                                         int64_t count, int B, int accumulate) {
@@ -1373,6 +1392,15 @@ int64_t rgf_grad_workspace(int K, const int32_t* bsz, int B) {
          inverse_workspace(nmax, B);
 }
 
+// bcorner_conjt_kernel over the B energies of an adjoint
+static hipError_t sig_corner(cx<double>* out, const cx<double>* X, int ld, int off, int m, cx<double> c,
+                             const double* w, int accumulate, int B, hipStream_t st) {
+  const int64_t tot = (int64_t)B * m * m;
+  hipLaunchKernelGGL(bcorner_conjt_kernel, dim3((int)std::min<int64_t>((tot + 255) / 256, 65535)),
+                     dim3(256), 0, st, out, X, ld, off, m, B, c, w, accumulate);
+  return hipGetLastError();
+}
+
 hipError_t rgf_grad(const RgfGradArgs& g, hipStream_t st) {
   const int K = g.K, B = g.B;
   const std::vector<int> n(g.bsz, g.bsz + K);
@@ -1452,6 +1480,11 @@ hipError_t rgf_grad(const RgfGradArgs& g, hipStream_t st) {
     CHK(gemm(st, B, nq, nq, nK, mat(lc(q), nK, (int64_t)nq * nK), mat(D[cur], nq, (int64_t)nK * nq),
              one, zero, Wm, (int64_t)nq * nq, nq, g.z));
     CHK(reduce(g.grad_hd + gsq[q], Wm, nq, nq));
+    // the self-energies enter M_0 / M_{K-1} as H does: their corners of
+    // (G_{q,K-1} D_q)^H, per energy
+    if (q == 0 && g.grad_sigL) CHK(sig_corner(g.grad_sigL, Wm, n0, 0, g.nL, one, nullptr, 0, B, st));
+    if (q == K - 1 && g.grad_sigR)
+      CHK(sig_corner(g.grad_sigR, Wm, nK, nK - g.nR, g.nR, one, nullptr, 0, B, st));
     if (q + 1 == K) break;
     const int n1 = n[q + 1];
     // R_{q+1} = (q == 0 ? g_0 : R_q g_q) U_q
@@ -1478,6 +1511,23 @@ hipError_t rgf_grad(const RgfGradArgs& g, hipStream_t st) {
              one, zero, Wm, (int64_t)nq * n1, n1, g.z));
     CHK(reduce(g.grad_hl + gx[q], Wm, nq, n1));
     cur = 1 - cur;
+  }
+  // and through Gamma: with Q = G_{0,K-1}, T = Tr[Gamma_L Q Gamma_R Q^H] and
+  // Gamma = i (Sigma - Sigma^H) add i 2 w (Q Gamma_R Q^H) to Sigma_L's and
+  // i (Q^H (2 w Gamma_L) Q) to Sigma_R's term of dJ = Re Tr[V dSigma]
+  if (g.grad_sigL) {
+    CHK(gemm(st, B, n0, nK, nK, mat(lc(0), nK, (int64_t)n0 * nK), mat(gamR, nK, (int64_t)nK * nK),
+             one, zero, Rb[0], (int64_t)n0 * nK, nK, g.z));
+    CHK(gemm(st, B, n0, n0, nK, mat(Rb[0], nK, (int64_t)n0 * nK),
+             mat(lc(0), nK, (int64_t)n0 * nK, OP_H), one, zero, G0[0], (int64_t)n0 * n0, n0, g.z));
+    CHK(sig_corner(g.grad_sigL, G0[0], n0, 0, g.nL, cx<double>{0, 1}, g.w, 1, B, st));
+  }
+  if (g.grad_sigR) {
+    CHK(gemm(st, B, nK, n0, n0, mat(lc(0), nK, (int64_t)n0 * nK, OP_H), mat(gamL, n0, (int64_t)n0 * n0),
+             one, zero, Rb[1], (int64_t)nK * n0, n0, g.z));
+    CHK(gemm(st, B, nK, nK, n0, mat(Rb[1], n0, (int64_t)nK * n0), mat(lc(0), nK, (int64_t)n0 * nK),
+             one, zero, G0[1], (int64_t)nK * nK, nK, g.z));
+    CHK(sig_corner(g.grad_sigR, G0[1], nK, nK - g.nR, g.nR, cx<double>{0, 1}, nullptr, 1, B, st));
   }
   return hipSuccess;
 }
@@ -1643,6 +1693,22 @@ hipError_t rgf_grad_ckpt(const RgfGradArgs& g, int stride, hipStream_t st) {
            (int64_t)nK * n0, n0, g.z));
   CHK(gemm(st, B, nK, n0, n0, M(Wm, n0, nK), M(gamL, n0, n0), one, zero, C, (int64_t)nK * n0, n0,
            g.z));
+  // the self-energies' terms through Gamma (as in rgf_grad), before t0..t3
+  // serve the sweep; their terms through G follow at q = 0 and q = K - 1
+  if (g.grad_sigL) {
+    CHK(gemm(st, B, n0, nK, nK, M(g0K, nK, n0), M(gamR, nK, nK), one, zero, t0, (int64_t)n0 * nK,
+             nK, g.z));
+    CHK(gemm(st, B, n0, n0, nK, M(t0, nK, n0), M(g0K, nK, n0, OP_H), one, zero, t1, (int64_t)n0 * n0,
+             n0, g.z));
+    CHK(sig_corner(g.grad_sigL, t1, n0, 0, g.nL, cx<double>{0, 1}, g.w, 0, B, st));
+  }
+  if (g.grad_sigR) {
+    CHK(gemm(st, B, nK, n0, n0, M(g0K, nK, n0, OP_H), M(gamL, n0, n0), one, zero, t0,
+             (int64_t)nK * n0, n0, g.z));
+    CHK(gemm(st, B, nK, nK, n0, M(t0, n0, nK), M(g0K, nK, n0), one, zero, t3, (int64_t)nK * nK, nK,
+             g.z));
+    CHK(sig_corner(g.grad_sigR, t3, nK, nK - g.nR, g.nR, cx<double>{0, 1}, nullptr, 0, B, st));
+  }
 
   // ---- gradient sweep, segment by segment from the right
   int cur = 0;  // carries of block q in [cur], of block q + 1 in [1 - cur]
@@ -1688,6 +1754,9 @@ hipError_t rgf_grad_ckpt(const RgfGradArgs& g, int stride, hipStream_t st) {
       CHK(gemm(st, B, nq, nq, nK, M(Gc[cur], nK, nq), M(D[cur], nq, nK), one, zero, Wm,
                (int64_t)nq * nq, nq, g.z));
       CHK(reduce(g.grad_hd + gsq[q], Wm, nq, nq));
+      if (q == 0 && g.grad_sigL) CHK(sig_corner(g.grad_sigL, Wm, n0, 0, g.nL, one, nullptr, 1, B, st));
+      if (q == K - 1 && g.grad_sigR)
+        CHK(sig_corner(g.grad_sigR, Wm, nK, nK - g.nR, g.nR, one, nullptr, 1, B, st));
       if (q + 1 < K) {
         const int n1 = n[q + 1];
         // grad_hu[q] = sum_b (G_{q+1,K-1} D_q)^H ;  grad_hl[q] = sum_b (G_{q,K-1} D_{q+1})^H

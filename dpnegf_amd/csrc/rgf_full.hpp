@@ -95,6 +95,8 @@ struct RgfGradArgs {
   cx<double>* grad_hd;         // [sum_q n_q^2]
   cx<double>* grad_hu;         // [sum_q n_q n_q+1]
   cx<double>* grad_hl;         // [sum_q n_q+1 n_q]
+  cx<double>* grad_sigL;       // [B, nL, nL] per energy, or null
+  cx<double>* grad_sigR;       // [B, nR, nR] per energy, or null
   cx<double>* work;            // rgf_grad_workspace elements
   int32_t* status;
 };

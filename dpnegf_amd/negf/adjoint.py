@@ -3,8 +3,9 @@
 ``transmission(...)`` returns T(E_b) for B energies of one system and is a
 ``torch.autograd.Function``: ``backward`` receives dJ/dT_b and calls
 ``dpnegf_rgf_transmission_grad``, which returns dJ/dH for every on-site block
-``hd_q`` and coupling block ``hu_q`` / ``hl_q`` summed over energies (torch's
-complex-gradient convention, dJ/dRe + i dJ/dIm).  The reference has no such
+``hd_q`` and coupling block ``hu_q`` / ``hl_q`` summed over energies, and
+dJ/dSigma_L, dJ/dSigma_R per energy (torch's complex-gradient convention,
+dJ/dRe + i dJ/dIm).  The reference has no such
 kernel; it relies on autograd through ``recursive_gf``
 (``dpnegf/negf/recursive_green_cal.py:89-112``), which is what the tests
 compare against, together with central finite differences.
@@ -52,6 +53,9 @@ class _Transmission(torch.autograd.Function):
         w = gT.detach().to(device=dev, dtype=torch.float64).contiguous()
         h_bsz = nat.host_i32(sizes)
         stride = checkpoint_stride(ctx.checkpoint, sizes, B, dev)
+        want_sig = ctx.needs_input_grad[2] or ctx.needs_input_grad[3]
+        gsL = torch.empty_like(sigL) if want_sig else None
+        gsR = torch.empty_like(sigR) if want_sig else None
         if stride:
             nwork = nat.load().dpnegf_rgf_grad_ckpt_workspace_elems(K, h_bsz, B, stride)
         else:
@@ -65,7 +69,7 @@ class _Transmission(torch.autograd.Function):
         status = torch.zeros(B, dtype=torch.int32, device=dev)
         args = (nat.stream_ptr(dev), K, h_bsz, nat.ptr(bs.arena), B, nat.ptr(z), nat.ptr(sigL),
                 sigL.shape[-1], nat.ptr(sigR), sigR.shape[-1], int(bs.ortho), nat.ptr(w),
-                nat.ptr(ghd), nat.ptr(ghu), nat.ptr(ghl))
+                nat.ptr(ghd), nat.ptr(ghu), nat.ptr(ghl), nat.ptr(gsL), nat.ptr(gsR))
         if stride:
             nat.call("dpnegf_rgf_transmission_grad_ckpt", *args, stride, nat.ptr(work),
                      nat.ptr(status))
@@ -75,7 +79,8 @@ class _Transmission(torch.autograd.Function):
         up = [(sizes[q], sizes[q + 1]) for q in range(K - 1)]
         g_hu = [t[0] for t in split_blocks(ghu, 1, up)]
         g_hl = [t[0] for t in split_blocks(ghl, 1, [(b, a) for a, b in up])]
-        return (None, None, None, None, *g_hd, *g_hu, *g_hl)
+        return (None, None, gsL if ctx.needs_input_grad[2] else None,
+                gsR if ctx.needs_input_grad[3] else None, *g_hd, *g_hu, *g_hl)
 
 
 def checkpoint_stride(checkpoint, sizes, B, dev, mem_fraction=0.5):
@@ -103,8 +108,9 @@ def transmission(energy, hd, hu, hl, sd, su, sl, sigL, sigR, eta, E_ref=0.0, pre
     """Differentiable T(E) [B] of one system.
 
     ``energy`` [B] relative to ``E_ref``; ``sigL`` [B, nL, nL] / ``sigR``
-    [B, nR, nR] the lead self-energies trimmed to their corners (treated as
-    constants).  Gradients flow to ``hd``, ``hu``, ``hl``.  ``checkpoint``:
+    [B, nR, nR] the lead self-energies trimmed to their corners.  Gradients
+    flow to ``hd``, ``hu``, ``hl`` (summed over the energies) and to ``sigL``,
+    ``sigR`` (per energy); the overlap blocks are constants.  ``checkpoint``:
     memory mode of the backward pass (``checkpoint_stride``).
     """
     dev = hd[0].device
@@ -114,5 +120,4 @@ def transmission(energy, hd, hu, hl, sd, su, sl, sigL, sigR, eta, E_ref=0.0, pre
     z = z.contiguous()
     meta = (K, [s.detach() for s in sd], [s.detach() for s in su], [s.detach() for s in sl],
             precision, checkpoint)
-    return _Transmission.apply(meta, z, sigL.detach().contiguous(), sigR.detach().contiguous(),
-                               *hd, *hu, *hl)
+    return _Transmission.apply(meta, z, sigL.contiguous(), sigR.contiguous(), *hd, *hu, *hl)

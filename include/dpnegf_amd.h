@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define DPNEGF_ABI_VERSION 6
+#define DPNEGF_ABI_VERSION 7
 
 #define DPNEGF_OK 0
 #define DPNEGF_ERR_ARG (-1)
@@ -241,11 +241,18 @@ int64_t dpnegf_rgf_grad_workspace_elems(int K, const int32_t* h_bsz, int B);
  * with the last block column from the backward sweep and the first block row
  * G_{0,b} = (g_0 U_0 ... g_{b-1} U_{b-1}) G_bb.
  *   grad_hd [sum n_q^2], grad_hu [sum n_q n_{q+1}], grad_hl [sum n_{q+1} n_q]
+ * and, per energy (not summed; either may be null), the gradients of the lead
+ * self-energies, which enter G as H does in the corners of the first and last
+ * blocks and enter T again through Gamma = i (Sigma - Sigma^H):
+ *   grad_sigL [B x nL x nL] = (W_00 + i 2 w Q Gamma_R Q^H)^H on the nL corner,
+ *   grad_sigR [B x nR x nR] = (W_{K-1,K-1} + i 2 w Q^H Gamma_L Q)^H on the nR
+ *   corner, Q = G_{0,K-1} restricted to the two corners.
  * Block sizes up to DPNEGF_FULL_NMAX; B <= 65535 per call. */
 int dpnegf_rgf_transmission_grad(void* stream, int K, const int32_t* h_bsz, const void* arena,
                                  int B, const void* z, const void* sigL, int nL, const void* sigR,
                                  int nR, int ortho, const double* w, void* grad_hd, void* grad_hu,
-                                 void* grad_hl, void* work, int32_t* status);
+                                 void* grad_hl, void* grad_sigL, void* grad_sigR, void* work,
+                                 int32_t* status);
 
 /* Complex elements of the work buffer dpnegf_rgf_transmission_grad_ckpt needs
  * for checkpoint stride `stride` (-1 for stride < 1): B times
@@ -264,8 +271,9 @@ int64_t dpnegf_rgf_grad_ckpt_workspace_elems(int K, const int32_t* h_bsz, int B,
 int dpnegf_rgf_transmission_grad_ckpt(void* stream, int K, const int32_t* h_bsz,
                                       const void* arena, int B, const void* z, const void* sigL,
                                       int nL, const void* sigR, int nR, int ortho, const double* w,
-                                      void* grad_hd, void* grad_hu, void* grad_hl, int stride,
-                                      void* work, int32_t* status);
+                                      void* grad_hd, void* grad_hu, void* grad_hl,
+                                      void* grad_sigL, void* grad_sigR, int stride, void* work,
+                                      int32_t* status);
 
 /* out[e] = (accumulate ? out[e] : 0) + sum_b Re(c[b] * in[b * count + e]):
  * the pole / quadrature sum of the Ozaki density, Ozaki.integrate

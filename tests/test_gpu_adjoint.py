@@ -180,3 +180,68 @@ def test_adjoint_blocks_above_128_matches_autograd(sizes):
         for r, g in zip(ref, got):
             err = (r - g.cpu()).abs().max().item() / scale
             assert err < 1e-9, (ck, err)
+
+
+@pytest.mark.parametrize("sizes,nL,nR", [([6, 9, 7, 8, 10], 4, 7), ([8], 5, 6)])
+def test_adjoint_self_energy_gradients(sizes, nL, nR):
+    """dJ/dSigma_L and dJ/dSigma_R per energy (Sigma trimmed to its corner,
+    smaller than the corner block; K = 1 puts both corners in one block) from
+    the adjoint, full storage and checkpointed, against autograd through the
+    oracle (the corner of the gradient of the zero-padded self-energy), to
+    1e-9 relative, and against central finite differences (delta 1e-6) on a
+    few entries.  Sigma enters both G and Gamma = i (Sigma - Sigma^H)."""
+    import torch.nn.functional as F
+    from dpnegf_amd.negf.adjoint import transmission
+    from tests.test_gpu_rgf import _rand_se
+    dev = _dev()
+    print(f"case sigma gradients sizes={sizes} nL={nL} nR={nR}", flush=True)
+    hd, sd, hl, hu, sl, su = wl.random_btd(sizes, seed=7, overlap=True)
+    B, eta = 3, 1e-3
+    E = [-0.7, 0.15, 0.6]
+    w = torch.tensor([0.8, -0.5, 1.2], dtype=torch.float64)
+    sL0, sR0 = _rand_se(B, nL, 11), _rand_se(B, nR, 12)
+    n0, nK = sizes[0], sizes[-1]
+    # autograd through the oracle, on the zero-padded corner self-energies
+    pL, pR = sL0.clone().requires_grad_(True), sR0.clone().requires_grad_(True)
+    fL = F.pad(pL, (0, n0 - nL, 0, n0 - nL))
+    fR = F.pad(pR, (nK - nR, 0, nK - nR, 0))
+    J = 0
+    for b, e in enumerate(E):
+        z = torch.tensor(e, dtype=torch.complex128)
+        ans = orgf.recursive_gf(z, hl, hd, hu, sd, su, sl, fL[b], fR[b], eta=eta,
+                                keep_gr_left=False)
+        J = J + w[b] * odev.transmission(ans[0], olead.sigma2gamma(fL[b]),
+                                         olead.sigma2gamma(fR[b]))
+    rL, rR = torch.autograd.grad(J, (pL, pR))
+    scale = max(rL.abs().max().item(), rR.abs().max().item())
+    to = lambda xs: [x.to(dev) for x in xs]
+    for ck in (0, 1, 3):
+        gL = sL0.to(dev).requires_grad_(True)
+        gR = sR0.to(dev).requires_grad_(True)
+        T = transmission(_f64(E), to(hd), to(hu), to(hl), to(sd), to(su), to(sl), gL, gR, eta,
+                         checkpoint=ck)
+        Jg = (T * w.to(dev)).sum()
+        assert abs(float(Jg.detach()) - float(J.detach())) < 1e-10 * max(1.0, abs(float(J)))
+        aL, aR = torch.autograd.grad(Jg, (gL, gR))
+        for name, r, a in (("sigL", rL, aL), ("sigR", rR, aR)):
+            err = (r - a.cpu()).abs().max().item() / scale
+            assert err < 1e-9, (ck, name, err)
+    # central finite differences on a few entries
+    delta = 1e-6
+    rng = np.random.default_rng(1)
+    for trial in range(4):
+        which = trial % 2
+        base = (sL0, sR0)[which]
+        b, i, j = (int(rng.integers(s)) for s in base.shape)
+        for direction in (1.0, 1j):
+            vals = []
+            for sgn in (1, -1):
+                pert = [sL0.clone(), sR0.clone()]
+                pert[which][b, i, j] += sgn * delta * direction
+                Tp = transmission(_f64(E), to(hd), to(hu), to(hl), to(sd), to(su), to(sl),
+                                  pert[0].to(dev), pert[1].to(dev), eta)
+                vals.append(float((Tp * w.to(dev)).sum()))
+            fd = (vals[0] - vals[1]) / (2 * delta)
+            g = (aL, aR)[which][b, i, j]
+            an = g.real.item() if direction == 1.0 else g.imag.item()
+            assert abs(fd - an) <= 1e-5 * max(abs(an), 1e-3 * scale), (which, b, i, j, fd, an)
