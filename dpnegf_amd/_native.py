@@ -57,10 +57,12 @@ _SIGS = {
                         _i32),
     "dpnegf_rgf_grad_workspace_elems": ([_i32, _vp, _i32], _i64),
     "dpnegf_rgf_transmission_grad": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32, _i32,
-                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i32),
+                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+                                     _i32),
     "dpnegf_rgf_grad_ckpt_workspace_elems": ([_i32, _vp, _i32, _i32], _i64),
     "dpnegf_rgf_transmission_grad_ckpt": ([_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _i32,
-                                           _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp],
+                                           _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                           _i32, _vp, _vp],
                                           _i32),
     "dpnegf_batch_sum_conjt": ([_vp, _vp, _vp, _i32, _i32, _i32, _i32], _i32),
     "dpnegf_weighted_re_sum": ([_vp, _vp, _vp, _vp, _i64, _i32, _i32], _i32),

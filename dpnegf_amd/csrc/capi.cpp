@@ -337,7 +337,8 @@ int64_t dpnegf_rgf_grad_workspace_elems(int K, const int32_t* h_bsz, int B) {
 static int grad_args(negf::RgfGradArgs& a, int K, const int32_t* h_bsz, const void* arena, int B,
                      const void* z, const void* sigL, int nL, const void* sigR, int nR, int ortho,
                      const double* w, void* grad_hd, void* grad_hu, void* grad_hl,
-                     void* grad_sigL, void* grad_sigR, void* work, int32_t* status) {
+                     void* grad_sigL, void* grad_sigR, void* grad_sd, void* grad_su,
+                     void* grad_sl, void* work, int32_t* status) {
   int nmax = 0;
   if (int r = check_blocks(K, h_bsz, DPNEGF_FULL_NMAX, &nmax)) return r;
   if (B < 0) return fail(DPNEGF_ERR_ARG, "B must be >= 0");
@@ -353,6 +354,10 @@ static int grad_args(negf::RgfGradArgs& a, int K, const int32_t* h_bsz, const vo
   a.sigR = (const cx<double>*)sigR; a.nR = nR; a.w = w; a.grad_hd = (cx<double>*)grad_hd;
   a.grad_hu = (cx<double>*)grad_hu; a.grad_hl = (cx<double>*)grad_hl;
   a.grad_sigL = (cx<double>*)grad_sigL; a.grad_sigR = (cx<double>*)grad_sigR;
+  a.grad_sd = (cx<double>*)grad_sd; a.grad_su = (cx<double>*)grad_su;
+  a.grad_sl = (cx<double>*)grad_sl;
+  if (K > 1 && grad_sd && (!grad_su || !grad_sl))
+    return fail(DPNEGF_ERR_ARG, "grad_su/grad_sl required with grad_sd");
   a.work = (cx<double>*)work;
   a.status = status;
   return 0;
@@ -361,11 +366,11 @@ static int grad_args(negf::RgfGradArgs& a, int K, const int32_t* h_bsz, const vo
 int dpnegf_rgf_transmission_grad(void* stream, int K, const int32_t* h_bsz, const void* arena,
                                  int B, const void* z, const void* sigL, int nL, const void* sigR,
                                  int nR, int ortho, const double* w, void* grad_hd, void* grad_hu,
-                                 void* grad_hl, void* grad_sigL, void* grad_sigR, void* work,
-                                 int32_t* status) {
+                                 void* grad_hl, void* grad_sigL, void* grad_sigR, void* grad_sd,
+                                 void* grad_su, void* grad_sl, void* work, int32_t* status) {
   negf::RgfGradArgs a;
   const int r = grad_args(a, K, h_bsz, arena, B, z, sigL, nL, sigR, nR, ortho, w, grad_hd, grad_hu,
-                          grad_hl, grad_sigL, grad_sigR, work, status);
+                          grad_hl, grad_sigL, grad_sigR, grad_sd, grad_su, grad_sl, work, status);
   if (r) return r < 0 ? r : 0;
   hipError_t e = negf::rgf_grad(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "rgf_transmission_grad");
@@ -381,12 +386,13 @@ int dpnegf_rgf_transmission_grad_ckpt(void* stream, int K, const int32_t* h_bsz,
                                       const void* arena, int B, const void* z, const void* sigL,
                                       int nL, const void* sigR, int nR, int ortho, const double* w,
                                       void* grad_hd, void* grad_hu, void* grad_hl,
-                                      void* grad_sigL, void* grad_sigR, int stride, void* work,
+                                      void* grad_sigL, void* grad_sigR, void* grad_sd,
+                                      void* grad_su, void* grad_sl, int stride, void* work,
                                       int32_t* status) {
   if (stride < 1) return fail(DPNEGF_ERR_ARG, "checkpoint stride %d < 1", stride);
   negf::RgfGradArgs a;
   const int r = grad_args(a, K, h_bsz, arena, B, z, sigL, nL, sigR, nR, ortho, w, grad_hd, grad_hu,
-                          grad_hl, grad_sigL, grad_sigR, work, status);
+                          grad_hl, grad_sigL, grad_sigR, grad_sd, grad_su, grad_sl, work, status);
   if (r) return r < 0 ? r : 0;
   hipError_t e = negf::rgf_grad_ckpt(a, stride, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "rgf_transmission_grad_ckpt");

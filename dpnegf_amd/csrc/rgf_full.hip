@@ -335,23 +335,25 @@ __global__ void badd_identity_kernel(cx<double>* D, int n, int B, const int32_t*
   }
 }
 
-// out[i][k] = sum_b conj(in[b][k][i]);  in: [B, m, n], out: [n, m]
-// out[i][k] = conj(sum_b in[b][k][i]) for an m x n block: a gradient block
-// summed over the batch, transposed and conjugated.  A 256-thread workgroup
+// out[i][k] = sign conj(sum_b c_b in[b][k][i]) (c null: c_b = 1) for an m x n
+// block: a gradient block summed over the batch, transposed and conjugated
+// (with c = z and sign -1, the overlap block's gradient from H's).  A 256-thread workgroup
 // takes RED_ELEMS consecutive elements; thread t sums element t % RED_ELEMS
 // over the batch entries b = t / RED_ELEMS + RED_PHASES j, so one wave reads
 // four 256-byte runs per step and a 64 x 64 block spreads over 256
 // workgroups.  The partial sums of an element are added in a fixed order.
 constexpr int RED_ELEMS = 16, RED_PHASES = 16;
 __global__ __launch_bounds__(256) void breduce_conjt_kernel(cx<double>* out, const cx<double>* in,
-                                                            int m, int n, int B, int accumulate) {
+                                                            int m, int n, int B, int accumulate,
+                                                            const cx<double>* c, double sign) {
   __shared__ cx<double> part[RED_PHASES][RED_ELEMS];
   const int64_t mn = (int64_t)m * n;
   const int el = threadIdx.x % RED_ELEMS, ph = threadIdx.x / RED_ELEMS;
   const int64_t e = (int64_t)blockIdx.x * RED_ELEMS + el;  // in element (k, i) = (e / n, e % n)
   cx<double> acc{0, 0};
   if (e < mn)
-    for (int b = ph; b < B; b += RED_PHASES) acc = acc + in[(int64_t)b * mn + e];
+    for (int b = ph; b < B; b += RED_PHASES)
+      acc = acc + (c ? c[b] * in[(int64_t)b * mn + e] : in[(int64_t)b * mn + e]);
   part[ph][el] = acc;
   __syncthreads();
   if (threadIdx.x < RED_ELEMS && e < mn) {
@@ -359,7 +361,8 @@ __global__ __launch_bounds__(256) void breduce_conjt_kernel(cx<double>* out, con
     for (int p = 1; p < RED_PHASES; ++p) sum = sum + part[p][el];
     const int k = (int)(e / n), i = (int)(e % n);
     const int64_t o = (int64_t)i * m + k;
-    out[o] = accumulate ? out[o] + conj(sum) : conj(sum);
+    const cx<double> r = {sign * sum.re, -sign * sum.im};
+    out[o] = accumulate ? out[o] + r : r;
   }
 }
 
@@ -490,7 +493,7 @@ hipError_t batch_sum_conjt(cx<double>* out, const cx<double>* in, int m, int n, 
                            int accumulate, hipStream_t st) {
   if (m <= 0 || n <= 0) return hipSuccess;
   hipLaunchKernelGGL(breduce_conjt_kernel, dim3(((int64_t)m * n + RED_ELEMS - 1) / RED_ELEMS),
-                     dim3(256), 0, st, out, in, m, n, B, accumulate);
+                     dim3(256), 0, st, out, in, m, n, B, accumulate, (const cx<double>*)nullptr, 1.0);
   return hipGetLastError();
 }
 
@@ -1466,7 +1469,15 @@ hipError_t rgf_grad(const RgfGradArgs& g, hipStream_t st) {
   // first block row: G_{0,0} = grd_0; G_{0,b} = R_b G_bb, R_1 = g_0 U_0, R_{b+1} = R_b g_b U_b
   auto reduce = [&](cx<double>* out, const cx<double>* in, int m_, int n_) {
     hipLaunchKernelGGL(breduce_conjt_kernel, dim3((m_ * n_ + RED_ELEMS - 1) / RED_ELEMS), dim3(256),
-                       0, st, out, in, m_, n_, B, 0);
+                       0, st, out, in, m_, n_, B, 0, (const cx<double>*)nullptr, 1.0);
+    return hipGetLastError();
+  };
+  // M = z S - H: the overlap block's gradient is -sum_b conj(z_b) (W_b)^H
+  // against H's sum_b (W_b)^H
+  auto reduce_s = [&](cx<double>* out, const cx<double>* in, int m_, int n_) -> hipError_t {
+    if (!out) return hipSuccess;
+    hipLaunchKernelGGL(breduce_conjt_kernel, dim3((m_ * n_ + RED_ELEMS - 1) / RED_ELEMS), dim3(256),
+                       0, st, out, in, m_, n_, B, 0, g.z, -1.0);
     return hipGetLastError();
   };
   const cx<double>* g0b = grd(0);  // G_{0,0}
@@ -1480,6 +1491,7 @@ hipError_t rgf_grad(const RgfGradArgs& g, hipStream_t st) {
     CHK(gemm(st, B, nq, nq, nK, mat(lc(q), nK, (int64_t)nq * nK), mat(D[cur], nq, (int64_t)nK * nq),
              one, zero, Wm, (int64_t)nq * nq, nq, g.z));
     CHK(reduce(g.grad_hd + gsq[q], Wm, nq, nq));
+    CHK(reduce_s(g.grad_sd ? g.grad_sd + gsq[q] : nullptr, Wm, nq, nq));
     // the self-energies enter M_0 / M_{K-1} as H does: their corners of
     // (G_{q,K-1} D_q)^H, per energy
     if (q == 0 && g.grad_sigL) CHK(sig_corner(g.grad_sigL, Wm, n0, 0, g.nL, one, nullptr, 0, B, st));
@@ -1507,9 +1519,11 @@ hipError_t rgf_grad(const RgfGradArgs& g, hipStream_t st) {
     CHK(gemm(st, B, n1, nq, nK, mat(lc(q + 1), nK, (int64_t)n1 * nK), mat(D[cur], nq, (int64_t)nK * nq),
              one, zero, Wm, (int64_t)n1 * nq, nq, g.z));
     CHK(reduce(g.grad_hu + gx[q], Wm, n1, nq));
+    CHK(reduce_s(g.grad_su ? g.grad_su + gx[q] : nullptr, Wm, n1, nq));
     CHK(gemm(st, B, nq, n1, nK, mat(lc(q), nK, (int64_t)nq * nK), mat(D[1 - cur], n1, (int64_t)nK * n1),
              one, zero, Wm, (int64_t)nq * n1, n1, g.z));
     CHK(reduce(g.grad_hl + gx[q], Wm, nq, n1));
+    CHK(reduce_s(g.grad_sl ? g.grad_sl + gx[q] : nullptr, Wm, nq, n1));
     cur = 1 - cur;
   }
   // and through Gamma: with Q = G_{0,K-1}, T = Tr[Gamma_L Q Gamma_R Q^H] and
@@ -1656,7 +1670,15 @@ hipError_t rgf_grad_ckpt(const RgfGradArgs& g, int stride, hipStream_t st) {
   };
   auto reduce = [&](cx<double>* out, const cx<double>* in, int m_, int n_) {
     hipLaunchKernelGGL(breduce_conjt_kernel, dim3((m_ * n_ + RED_ELEMS - 1) / RED_ELEMS), dim3(256),
-                       0, st, out, in, m_, n_, B, 0);
+                       0, st, out, in, m_, n_, B, 0, (const cx<double>*)nullptr, 1.0);
+    return hipGetLastError();
+  };
+  // M = z S - H: the overlap block's gradient is -sum_b conj(z_b) (W_b)^H
+  // against H's sum_b (W_b)^H
+  auto reduce_s = [&](cx<double>* out, const cx<double>* in, int m_, int n_) -> hipError_t {
+    if (!out) return hipSuccess;
+    hipLaunchKernelGGL(breduce_conjt_kernel, dim3((m_ * n_ + RED_ELEMS - 1) / RED_ELEMS), dim3(256),
+                       0, st, out, in, m_, n_, B, 0, g.z, -1.0);
     return hipGetLastError();
   };
 
@@ -1754,6 +1776,7 @@ hipError_t rgf_grad_ckpt(const RgfGradArgs& g, int stride, hipStream_t st) {
       CHK(gemm(st, B, nq, nq, nK, M(Gc[cur], nK, nq), M(D[cur], nq, nK), one, zero, Wm,
                (int64_t)nq * nq, nq, g.z));
       CHK(reduce(g.grad_hd + gsq[q], Wm, nq, nq));
+      CHK(reduce_s(g.grad_sd ? g.grad_sd + gsq[q] : nullptr, Wm, nq, nq));
       if (q == 0 && g.grad_sigL) CHK(sig_corner(g.grad_sigL, Wm, n0, 0, g.nL, one, nullptr, 1, B, st));
       if (q == K - 1 && g.grad_sigR)
         CHK(sig_corner(g.grad_sigR, Wm, nK, nK - g.nR, g.nR, one, nullptr, 1, B, st));
@@ -1763,9 +1786,11 @@ hipError_t rgf_grad_ckpt(const RgfGradArgs& g, int stride, hipStream_t st) {
         CHK(gemm(st, B, n1, nq, nK, M(Gc[1 - cur], nK, n1), M(D[cur], nq, nK), one, zero, Wm,
                  (int64_t)n1 * nq, nq, g.z));
         CHK(reduce(g.grad_hu + gx[q], Wm, n1, nq));
+        CHK(reduce_s(g.grad_su ? g.grad_su + gx[q] : nullptr, Wm, n1, nq));
         CHK(gemm(st, B, nq, n1, nK, M(Gc[cur], nK, nq), M(D[1 - cur], n1, nK), one, zero, Wm,
                  (int64_t)nq * n1, n1, g.z));
         CHK(reduce(g.grad_hl + gx[q], Wm, nq, n1));
+        CHK(reduce_s(g.grad_sl ? g.grad_sl + gx[q] : nullptr, Wm, nq, n1));
       }
       cur = 1 - cur;
     }

@@ -97,6 +97,9 @@ struct RgfGradArgs {
   cx<double>* grad_hl;         // [sum_q n_q+1 n_q]
   cx<double>* grad_sigL;       // [B, nL, nL] per energy, or null
   cx<double>* grad_sigR;       // [B, nR, nR] per energy, or null
+  cx<double>* grad_sd;         // [sum_q n_q^2], or null (the overlap blocks)
+  cx<double>* grad_su;         // [sum_q n_q n_q+1], or null
+  cx<double>* grad_sl;         // [sum_q n_q+1 n_q], or null
   cx<double>* work;            // rgf_grad_workspace elements
   int32_t* status;
 };

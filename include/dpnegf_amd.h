@@ -246,13 +246,17 @@ int64_t dpnegf_rgf_grad_workspace_elems(int K, const int32_t* h_bsz, int B);
  * blocks and enter T again through Gamma = i (Sigma - Sigma^H):
  *   grad_sigL [B x nL x nL] = (W_00 + i 2 w Q Gamma_R Q^H)^H on the nL corner,
  *   grad_sigR [B x nR x nR] = (W_{K-1,K-1} + i 2 w Q^H Gamma_L Q)^H on the nR
- *   corner, Q = G_{0,K-1} restricted to the two corners.
+ *   corner, Q = G_{0,K-1} restricted to the two corners;
+ * and (any may be null; grad_su / grad_sl required with grad_sd when K > 1)
+ * the gradients of the overlap blocks, which enter only through M = z S - H:
+ *   grad_sd / grad_su / grad_sl = -sum_b conj(z_b) (W_b)^H on each block,
+ * laid out as grad_hd / grad_hu / grad_hl.
  * Block sizes up to DPNEGF_FULL_NMAX; B <= 65535 per call. */
 int dpnegf_rgf_transmission_grad(void* stream, int K, const int32_t* h_bsz, const void* arena,
                                  int B, const void* z, const void* sigL, int nL, const void* sigR,
                                  int nR, int ortho, const double* w, void* grad_hd, void* grad_hu,
-                                 void* grad_hl, void* grad_sigL, void* grad_sigR, void* work,
-                                 int32_t* status);
+                                 void* grad_hl, void* grad_sigL, void* grad_sigR, void* grad_sd,
+                                 void* grad_su, void* grad_sl, void* work, int32_t* status);
 
 /* Complex elements of the work buffer dpnegf_rgf_transmission_grad_ckpt needs
  * for checkpoint stride `stride` (-1 for stride < 1): B times
@@ -272,7 +276,8 @@ int dpnegf_rgf_transmission_grad_ckpt(void* stream, int K, const int32_t* h_bsz,
                                       const void* arena, int B, const void* z, const void* sigL,
                                       int nL, const void* sigR, int nR, int ortho, const double* w,
                                       void* grad_hd, void* grad_hu, void* grad_hl,
-                                      void* grad_sigL, void* grad_sigR, int stride, void* work,
+                                      void* grad_sigL, void* grad_sigR, void* grad_sd,
+                                      void* grad_su, void* grad_sl, int stride, void* work,
                                       int32_t* status);
 
 /* out[e] = (accumulate ? out[e] : 0) + sum_b Re(c[b] * in[b * count + e]):

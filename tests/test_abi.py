@@ -114,8 +114,8 @@ def test_checkpointed_adjoint_workspace():
     assert ck * 4 < full
     assert lib.dpnegf_rgf_grad_ckpt_workspace_elems(64, h, 16, 0) == -1
     rc = lib.dpnegf_rgf_transmission_grad_ckpt(None, 1, nat.host_i32([4]), None, 1, None, None, 1,
-                                               None, 1, 1, None, None, None, None, None, None, 0,
-                                               None, None)
+                                               None, 1, 1, None, None, None, None, None, None,
+                                               None, None, None, 0, None, None)
     assert rc < 0 and b"stride" in lib.dpnegf_last_error()
 
 

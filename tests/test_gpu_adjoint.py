@@ -185,11 +185,12 @@ def test_adjoint_blocks_above_128_matches_autograd(sizes):
 @pytest.mark.parametrize("sizes,nL,nR", [([6, 9, 7, 8, 10], 4, 7), ([8], 5, 6)])
 def test_adjoint_self_energy_gradients(sizes, nL, nR):
     """dJ/dSigma_L and dJ/dSigma_R per energy (Sigma trimmed to its corner,
-    smaller than the corner block; K = 1 puts both corners in one block) from
-    the adjoint, full storage and checkpointed, against autograd through the
-    oracle (the corner of the gradient of the zero-padded self-energy), to
-    1e-9 relative, and against central finite differences (delta 1e-6) on a
-    few entries.  Sigma enters both G and Gamma = i (Sigma - Sigma^H)."""
+    smaller than the corner block; K = 1 puts both corners in one block) and
+    dJ/dS for the overlap blocks from the adjoint, full storage and
+    checkpointed, against autograd through the oracle (for Sigma the corner of
+    the gradient of the zero-padded self-energy), to 1e-9 relative, and
+    Sigma's against central finite differences (delta 1e-6) on a few entries.
+    Sigma enters both G and Gamma = i (Sigma - Sigma^H); S only M = z S - H."""
     import torch.nn.functional as F
     from dpnegf_amd.negf.adjoint import transmission
     from tests.test_gpu_rgf import _rand_se
@@ -205,27 +206,35 @@ def test_adjoint_self_energy_gradients(sizes, nL, nR):
     pL, pR = sL0.clone().requires_grad_(True), sR0.clone().requires_grad_(True)
     fL = F.pad(pL, (0, n0 - nL, 0, n0 - nL))
     fR = F.pad(pR, (nK - nR, 0, nK - nR, 0))
+    p_s = [x.clone().requires_grad_(True) for x in sd + su + sl]
+    K = len(sizes)
+    o_sd, o_su, o_sl = p_s[:K], p_s[K:2 * K - 1], p_s[2 * K - 1:]
     J = 0
     for b, e in enumerate(E):
         z = torch.tensor(e, dtype=torch.complex128)
-        ans = orgf.recursive_gf(z, hl, hd, hu, sd, su, sl, fL[b], fR[b], eta=eta,
+        ans = orgf.recursive_gf(z, hl, hd, hu, o_sd, o_su, o_sl, fL[b], fR[b], eta=eta,
                                 keep_gr_left=False)
         J = J + w[b] * odev.transmission(ans[0], olead.sigma2gamma(fL[b]),
                                          olead.sigma2gamma(fR[b]))
-    rL, rR = torch.autograd.grad(J, (pL, pR))
+    rL, rR, *r_s = torch.autograd.grad(J, [pL, pR] + p_s)
     scale = max(rL.abs().max().item(), rR.abs().max().item())
+    scale_s = max(r.abs().max().item() for r in r_s)
     to = lambda xs: [x.to(dev) for x in xs]
     for ck in (0, 1, 3):
         gL = sL0.to(dev).requires_grad_(True)
         gR = sR0.to(dev).requires_grad_(True)
-        T = transmission(_f64(E), to(hd), to(hu), to(hl), to(sd), to(su), to(sl), gL, gR, eta,
-                         checkpoint=ck)
+        g_s = [x.to(dev).requires_grad_(True) for x in sd + su + sl]
+        T = transmission(_f64(E), to(hd), to(hu), to(hl), g_s[:K], g_s[K:2 * K - 1],
+                         g_s[2 * K - 1:], gL, gR, eta, checkpoint=ck)
         Jg = (T * w.to(dev)).sum()
         assert abs(float(Jg.detach()) - float(J.detach())) < 1e-10 * max(1.0, abs(float(J)))
-        aL, aR = torch.autograd.grad(Jg, (gL, gR))
+        aL, aR, *a_s = torch.autograd.grad(Jg, [gL, gR] + g_s)
         for name, r, a in (("sigL", rL, aL), ("sigR", rR, aR)):
             err = (r - a.cpu()).abs().max().item() / scale
             assert err < 1e-9, (ck, name, err)
+        for q, (r, a) in enumerate(zip(r_s, a_s)):
+            err = (r - a.cpu()).abs().max().item() / scale_s
+            assert err < 1e-9, (ck, "S block", q, err)
     # central finite differences on a few entries
     delta = 1e-6
     rng = np.random.default_rng(1)
